@@ -1,0 +1,235 @@
+/*
+ * mmcsf.h -- C ABI of the MI355X (gfx950) sparse-MTTKRP library.
+ *
+ * Drop-in boundary for the MM-CSF reference's MTTKRP path.  Every entry point
+ * names the reference function it replaces (paths relative to the reference
+ * checkout).  Conventions:
+ *   - every function returns int: 0 = ok, negative = error; the message of the
+ *     last error on the calling thread is mmcsf_last_error();
+ *   - nothing calls exit() and nothing mutates caller-owned inputs;
+ *   - device pointers are HIP device addresses, streams are hipStream_t passed
+ *     as void* (NULL = the null stream); every device call is stream-ordered;
+ *   - a handle may be used from one host thread at a time;
+ *   - tensor coordinates are 0-based; device index arrays are uint32 (every
+ *     count and coordinate of a supported tensor is < 2^31); host-side
+ *     exports are widened to uint64 like the reference's ITYPE (size_t).
+ */
+#ifndef MMCSF_H
+#define MMCSF_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes --------------------------------------------------------- */
+#define MMCSF_OK 0
+#define MMCSF_ERR_ARG (-1)        /* bad argument / option */
+#define MMCSF_ERR_HIP (-2)        /* HIP runtime error */
+#define MMCSF_ERR_IO (-3)         /* file could not be read / written */
+#define MMCSF_ERR_UNSUPPORTED (-4)/* valid request outside the supported range */
+#define MMCSF_ERR_NOMEM (-5)      /* device or host allocation failed */
+
+/* ---- format types: the reference's -t values (mttkrp.cu:104-417) ---------- */
+#define MMCSF_TYPE_COO_CPU 1      /* -t 1  COO on the host (sequential float) */
+#define MMCSF_TYPE_COO 3          /* -t 3  COO on the GPU */
+#define MMCSF_TYPE_BCSF 8         /* -t 8  B-CSF (tiled HCSR + slice bins) */
+#define MMCSF_TYPE_HBCSF 10       /* -t 10 HB-CSF (COO + CSL + CSF parts) */
+#define MMCSF_TYPE_MMCSF 12       /* -t 12 MM-CSF, every mode from one build */
+
+/* ---- factor storage --------------------------------------------------------- */
+#define MMCSF_F32 0               /* float32 rows */
+#define MMCSF_BF16 1              /* bfloat16 rows (uint16 bit patterns), FP32 math */
+
+/*
+ * Options -- the reference's Options class (util.h:189-236), same defaults.
+ * Parsed by mmcsf_options_parse with the grammar of parse_cmd_options
+ * (util.h:3341-3442).
+ */
+typedef struct mmcsf_options {
+    uint64_t R;               /* -R rank (32) */
+    uint64_t mode;            /* -m MTTKRP mode (0) */
+    uint64_t impType;         /* -t implementation type (1) */
+    uint64_t warpPerSlice;    /* -w (4) */
+    uint64_t nTile;           /* -l (1; tiling is forced off, util.h:724-727) */
+    uint64_t gridSize;        /* -g (512) */
+    uint64_t TBsize;          /* -b (128) */
+    uint64_t MIfbTh;          /* -h (1) */
+    uint64_t fiberPerWarp;    /* -s (1) */
+    int verbose;              /* -v 1 */
+    int correctness;          /* -c 1 */
+    uint64_t nBin;            /* 10 (no flag) */
+    uint64_t cpdIters;        /* 10 (no flag in the reference) */
+    int natOrdering;          /* 0 (no flag) */
+    uint64_t fbrThreashold;   /* -f fiber split threshold (99999999) */
+    int factorDtype;          /* -d 0|1: FP32 / BF16 factor storage (new flag) */
+    char inFileName[4096];    /* -i */
+    char outFileName[4096];   /* -o */
+    char m0[16], m1[16], m2[16];  /* -p/-q/-r, unused like the reference */
+} mmcsf_options;
+
+/* Opaque handles. */
+typedef struct mmcsf_tensor mmcsf_tensor;  /* device COO: class Tensor, util.h:80-108 */
+typedef struct mmcsf_format mmcsf_format;  /* device COO / B-CSF / HB-CSF / MM-CSF:
+                                              TiledTensor + HYBTensor, util.h:110-173 */
+
+/* Last error message of the calling thread ("" if none). */
+const char *mmcsf_last_error(void);
+
+/* Options() defaults, util.h:189-211. */
+int mmcsf_options_default(mmcsf_options *opt);
+
+/* parse_cmd_options, util.h:3341-3442: "-X value" pairs, parsing stops at the
+ * first token not starting with '-'.  Returns MMCSF_ERR_ARG (instead of the
+ * reference's help-and-exit(1)) on an unknown flag or a flag without value. */
+int mmcsf_options_parse(int argc, char **argv, mmcsf_options *opt);
+
+/* ---- tensors ---------------------------------------------------------------- */
+
+/* load_tensor, util.h:313-370: .tns text (ndims, dims, then 1-based
+ * coordinates and a value per line) -> device COO with 0-based coordinates.
+ * Errors (instead of exit(0)) on a missing file or ndims outside 3..4. */
+int mmcsf_tensor_load_tns(const char *path, mmcsf_tensor **out);
+
+/* Device COO from host arrays in the reference layout: inds[m][x] (size_t
+ * per mode, as Tensor::inds, util.h:89) and vals[x] (float). */
+int mmcsf_tensor_from_host(int ndims, const uint64_t *dims, uint64_t nnz,
+                           const uint64_t *const *inds, const float *vals,
+                           mmcsf_tensor **out);
+
+/* Device COO from device arrays (copied; uint32 coordinates). */
+int mmcsf_tensor_from_device(int ndims, const uint64_t *dims, uint64_t nnz,
+                             const uint32_t *const *inds_dev,
+                             const float *vals_dev, void *stream,
+                             mmcsf_tensor **out);
+
+/* Synthetic tensor generated on the GPU (no reference counterpart; the
+ * reference reads FROSTT files).  law[m]: 0 = uniform, 1 = Zipf(alpha[m])
+ * with hot rows at low indices.  Coordinates are distinct (accepted in draw
+ * order, repeats rejected).  Values U(0,1) float32.  Deterministic in seed. */
+int mmcsf_tensor_generate(int ndims, const uint64_t *dims, uint64_t nnz,
+                          const int *law, const double *alpha, uint64_t seed,
+                          void *stream, mmcsf_tensor **out);
+
+int mmcsf_tensor_info(const mmcsf_tensor *t, int *ndims, uint64_t *dims,
+                      uint64_t *nnz);
+/* Device views of the COO arrays (owned by the handle). */
+int mmcsf_tensor_device_arrays(const mmcsf_tensor *t, const uint32_t **inds,
+                               const float **vals);
+int mmcsf_tensor_copy_to_host(const mmcsf_tensor *t, uint64_t *const *inds,
+                              float *vals);
+/* Stream-ordered copy of the COO into caller device buffers (uint32
+ * coordinates per mode, float values); NULL entries are skipped. */
+int mmcsf_tensor_copy_to_device(const mmcsf_tensor *t, uint32_t *const *inds,
+                                float *vals, void *stream);
+void mmcsf_tensor_free(mmcsf_tensor *t);
+
+/* ---- formats (GPU builder) -------------------------------------------------- */
+
+/*
+ * Builds a device format with GPU sort / scan / segment passes.  Reference
+ * flows (mttkrp.cu):
+ *   MMCSF_TYPE_COO   sort_COOtensor (util.h:412-475)
+ *   MMCSF_TYPE_BCSF  create_HCSR + make_KTiling + create_TiledHCSR +
+ *                    make_TiledBin + create_fbrLikeSlcInds (mttkrp.cu:153-219)
+ *   MMCSF_TYPE_HBCSF create_HCSR + create_HYB[_4D] + make_HybBin
+ *                    (mttkrp.cu:138-150, util.h:852-1142, 1786-1885)
+ *   MMCSF_TYPE_MMCSF mm_partition_reuseBased + populate_paritions +
+ *                    sort_MI_CSF + create_TiledHCSR + create_fbrLikeSlcInds
+ *                    (mttkrp.cu:289-395, util.h:2088-2526)
+ * Uses opt->mode (root mode of COO/B-CSF/HB-CSF) and opt->fbrThreashold.
+ * Index/pointer arrays equal the reference's bit for bit (see
+ * mmcsf_format_export); the order of nonzeros inside a fiber is the
+ * canonical full-key order, one of the orders the reference may produce.
+ */
+int mmcsf_format_build(const mmcsf_tensor *t, int type,
+                       const mmcsf_options *opt, void *stream,
+                       mmcsf_format **out);
+
+/* Summary of a built format. */
+typedef struct mmcsf_format_info {
+    int type;
+    int ndims;
+    uint64_t dims[4];
+    uint64_t totNnz;
+    int modeOrder[4];          /* of the (non-empty) partition / the format */
+    uint64_t nLevel[4];        /* entries per CSF level (level ndims-1 = nnz) */
+    uint64_t nFibers;          /* reference convention: fbrPtr[1].size() */
+    uint64_t droppedSlices;    /* slices the reference's binning drops */
+    uint64_t COOnnz, CSLnnz, HCSRnnz;  /* HB-CSF parts */
+    int redunMode;             /* mttkrp.cu:315-347 */
+} mmcsf_format_info;
+
+int mmcsf_format_info_get(const mmcsf_format *f, mmcsf_format_info *info);
+
+/*
+ * Copies one reference field to host memory, widened to uint64 (float fields
+ * are written as float).  `name` is the reference's member name; `k` selects
+ * the level / bin / mode / partition where the member is a vector of
+ * vectors.  Fields (sizes follow the reference's vectors exactly):
+ *   "fbrPtr" k, "fbrIdx" k (level k; the leaf-fiber level carries the
+ *   reference's trailing entry), "inds" k (coordinates of tensor mode k in
+ *   stored order), "vals" (float), "fbrLikeSlcInds" (nFibers entries, the
+ *   last one unspecified), "slcMapperBin" k, "partPerNnz",
+ *   HB-CSF: "COOinds" k (tensor mode k), "COOvals", "CSLslicePtr",
+ *   "CSLsliceIdx", "CSLinds" k (k = mode1 / mode2 of modeOrder),
+ *   "CSLvals", "CSLslcMapperBin" k.
+ * *len receives the element count; nothing is written when cap < *len.
+ */
+int mmcsf_format_export(const mmcsf_format *f, const char *name, int k,
+                        void *dst, uint64_t cap, uint64_t *len);
+
+void mmcsf_format_free(mmcsf_format *f);
+
+/* ---- MTTKRP ------------------------------------------------------------------ */
+
+/*
+ * out = MTTKRP of the format on tensor mode `mode`:
+ *   out[i][r] = sum over nonzeros x with coordinate i in `mode` of
+ *               v_x * prod_{m != mode} U[m][i_m(x)][r].
+ * factors[m]: device pointer to U[m], row-major dims[m] x R, float32 or
+ * bfloat16 per `dtype`; factors[mode] is not read.  out: device float32
+ * dims[mode] x R, overwritten.  Accumulation is float32.
+ * Replaces (by format type):
+ *   COO    MTTKRP_COO_GPU            (mttkrp_gpu.h:61, mttkrp_gpu.cu:1562)
+ *   B-CSF  MTTKRP_B_HCSR_GPU         (mttkrp_gpu.h:67, mttkrp_gpu.cu:1850)
+ *   HB-CSF MTTKRP_HYB_GPU            (mttkrp_gpu.h:73, mttkrp_gpu.cu:2530)
+ *   MM-CSF MTTKRP_MIHCSR_GPU[_oneMode_forCPD]
+ *                                    (mttkrp_gpu.h:75-81, mttkrp_gpu.cu:3380, 3860)
+ * COO, B-CSF and HB-CSF serve their root mode (modeOrder[0]) only, like
+ * the reference; MM-CSF serves every mode.
+ */
+int mmcsf_mttkrp(const mmcsf_format *f, int mode, int R, int dtype,
+                 const void *const *factors, float *out, void *stream);
+
+/*
+ * All-mode MM-CSF MTTKRP (MTTKRP_MIHCSR_GPU's mode loop,
+ * mttkrp_gpu.cu:3548-3744): outs[m] receives the MTTKRP of mode m for every
+ * m in 0..ndims-1, all from the same factors.  Optional ms_per_mode (host,
+ * ndims entries) receives each mode's MTTKRP kernel time (after the output
+ * has been zeroed), measured with HIP events on `stream`; the call then
+ * synchronises on the last event (NULL: no timing, no synchronisation).
+ */
+int mmcsf_mttkrp_allmode(const mmcsf_format *f, int R, int dtype,
+                         const void *const *factors, float *const *outs,
+                         void *stream, float *ms_per_mode);
+
+/* Sequential float32 COO MTTKRP on the host (-t 1, MTTKRP_COO_CPU[_4D],
+ * mttkrp_cpu.cpp:60-113) over a format of type MMCSF_TYPE_COO_CPU or
+ * MMCSF_TYPE_COO (the nonzeros sorted as sort_COOtensor leaves them,
+ * mttkrp.cu:79): cyclic mode roles, stored order, accumulated into out (not
+ * zeroed).  Host factor and output pointers.  This is the reference's CPU
+ * type, selected explicitly; no GPU entry point ever falls back to it. */
+int mmcsf_mttkrp_coo_host(const mmcsf_format *f, int mode, int R,
+                          const float *const *factors_host, float *out_host);
+
+/* ---- version ---------------------------------------------------------------- */
+const char *mmcsf_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MMCSF_H */

@@ -1,0 +1,634 @@
+// C ABI of the library (include/mmcsf.h).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cerrno>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "mmcsf_internal.hpp"
+
+namespace mmcsf {
+
+static thread_local std::string g_last_error;
+
+void set_last_error(const std::string &msg) { g_last_error = msg; }
+
+namespace {
+
+template <typename F>
+int guarded(F &&fn) {
+    try {
+        fn();
+        return MMCSF_OK;
+    } catch (const Error &e) {
+        set_last_error(e.what());
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        set_last_error("host allocation failed");
+        return MMCSF_ERR_NOMEM;
+    } catch (const std::exception &e) {
+        set_last_error(e.what());
+        return MMCSF_ERR_ARG;
+    }
+}
+
+constexpr uint64_t kMaxIndex = 0x7fffffffull;  // device coordinates and counts are uint32 < 2^31
+
+void check_shape(int ndims, const uint64_t *dims, uint64_t nnz) {
+    // check_opt, util.h:238-243 (the reference accepts 3 or 4 modes)
+    MMCSF_REQUIRE(ndims == 3 || ndims == 4, MMCSF_ERR_UNSUPPORTED, "Supported tensor dimension is 3 or 4.");
+    for (int m = 0; m < ndims; ++m)
+        MMCSF_REQUIRE(dims[m] >= 1 && dims[m] <= kMaxIndex, MMCSF_ERR_UNSUPPORTED,
+                      "mode length must be in 1..2^31-1");
+    MMCSF_REQUIRE(nnz <= kMaxIndex, MMCSF_ERR_UNSUPPORTED, "nnz must be below 2^31");
+}
+
+__global__ void k_u64_to_u32_check(const uint64_t *in, uint32_t *out, uint64_t n, uint64_t lim,
+                                   uint32_t *bad) {
+    uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint64_t v = in[i];
+    if (v >= lim) atomicOr(bad, 1u);
+    out[i] = (uint32_t)v;
+}
+
+__global__ void k_u32_check(const uint32_t *in, uint64_t n, uint64_t lim, uint32_t *bad) {
+    uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (i < n && in[i] >= lim) atomicOr(bad, 1u);
+}
+
+void upload_tensor(int ndims, const uint64_t *dims, uint64_t nnz, const std::vector<std::vector<uint32_t>> &inds,
+                   const float *vals, Tensor &t) {
+    t.ndims = ndims;
+    t.nnz = nnz;
+    for (int m = 0; m < ndims; ++m) {
+        t.dims[m] = dims[m];
+        t.inds[m].alloc(nnz);
+        if (nnz) MMCSF_HIP(hipMemcpy(t.inds[m].p, inds[m].data(), nnz * 4, hipMemcpyHostToDevice));
+    }
+    t.vals.alloc(nnz);
+    if (nnz) MMCSF_HIP(hipMemcpy(t.vals.p, vals, nnz * 4, hipMemcpyHostToDevice));
+}
+
+int level_of(const int *order, int ndims, int mode) {
+    for (int l = 0; l < ndims; ++l)
+        if (order[l] == mode) return l;
+    return -1;
+}
+
+template <typename T>
+std::vector<T> to_host(const DevArray<T> &d, uint64_t n) {
+    std::vector<T> h(n);
+    if (n) MMCSF_HIP(hipMemcpy(h.data(), d.p, n * sizeof(T), hipMemcpyDeviceToHost));
+    return h;
+}
+
+// Coordinates of level l for every nonzero of a CSF tree (expands the tree).
+std::vector<uint32_t> expand_level(const Csf &c, int l) {
+    const int N = c.ndims;
+    if (l == N - 1) return to_host(c.leaf, c.nnz);
+    std::vector<uint32_t> out(c.nnz);
+    std::vector<std::vector<uint32_t>> ptr(N - 1);
+    for (int q = 0; q < N - 1; ++q) ptr[q] = to_host(c.ptr[q], c.cnt[q] + 1);
+    std::vector<uint32_t> idx = to_host(c.idx[l], c.cnt[l]);
+    for (uint64_t e = 0; e < c.cnt[l]; ++e) {
+        uint64_t lo = e, hi = e + 1;
+        for (int q = l; q < N - 1; ++q) { lo = ptr[q][lo]; hi = ptr[q][hi]; }
+        for (uint64_t x = lo; x < hi; ++x) out[x] = idx[e];
+    }
+    return out;
+}
+
+bool mmcsf_type_ok(int type) {
+    return type == MMCSF_TYPE_COO_CPU || type == MMCSF_TYPE_COO || type == MMCSF_TYPE_BCSF ||
+           type == MMCSF_TYPE_HBCSF || type == MMCSF_TYPE_MMCSF;
+}
+
+int redundant_mode(int ndims, const uint64_t *dims) {
+    // mttkrp.cu:315-347: the last m whose (order[0], order[1]) swaps an
+    // earlier mode's pair
+    int orders[kMaxDims][kMaxDims];
+    for (int m = 0; m < ndims; ++m) order_tensormode(ndims, dims, m, false, orders[m]);
+    int red = -1;
+    for (int m = 1; m < ndims; ++m)
+        for (int q = 0; q < m; ++q)
+            if (orders[m][1] == orders[q][0] && orders[m][0] == orders[q][1]) red = m;
+    return red;
+}
+
+}  // namespace
+}  // namespace mmcsf
+
+using namespace mmcsf;
+
+extern "C" {
+
+const char *mmcsf_last_error(void) { return g_last_error.c_str(); }
+
+const char *mmcsf_version(void) { return "mmcsf-mi355x 0.1 (gfx950)"; }
+
+int mmcsf_options_default(mmcsf_options *o) {
+    if (!o) { set_last_error("null options"); return MMCSF_ERR_ARG; }
+    std::memset(o, 0, sizeof(*o));
+    // Options member initialisers, util.h:189-211
+    o->R = 32;
+    o->mode = 0;
+    o->impType = 1;
+    o->warpPerSlice = 4;
+    o->nTile = 1;
+    o->gridSize = 512;
+    o->TBsize = 128;
+    o->MIfbTh = 1;
+    o->fiberPerWarp = 1;
+    o->verbose = 0;
+    o->correctness = 0;
+    o->nBin = 10;
+    o->cpdIters = 10;
+    o->natOrdering = 0;
+    o->fbrThreashold = 99999999;
+    o->factorDtype = MMCSF_F32;
+    std::strcpy(o->m0, "012");
+    std::strcpy(o->m1, "120");
+    std::strcpy(o->m2, "201");
+    return MMCSF_OK;
+}
+
+int mmcsf_options_parse(int argc, char **argv, mmcsf_options *o) {
+    return guarded([&] {
+        MMCSF_REQUIRE(o != nullptr, MMCSF_ERR_ARG, "null options");
+        mmcsf_options_default(o);
+        // parse_cmd_options, util.h:3341-3442.  atoi semantics: ITYPE fields
+        // receive the int converted to size_t.
+        auto as_u64 = [](const char *s) { return (uint64_t)(int64_t)std::atoi(s); };
+        int i;
+        for (i = 1; i < argc; i++) {
+            if (argv[i][0] != '-') break;
+            if (++i >= argc) fail(MMCSF_ERR_ARG, std::string("option ") + argv[i - 1] + " requires a value");
+            const char *v = argv[i];
+            switch (argv[i - 1][1]) {
+                case 'R': o->R = as_u64(v); break;
+                case 'm': o->mode = as_u64(v); break;
+                case 't': o->impType = as_u64(v); break;
+                case 'w': o->warpPerSlice = as_u64(v); break;
+                case 'l': o->nTile = as_u64(v); break;
+                case 'f': o->fbrThreashold = as_u64(v); break;
+                case 'b': o->TBsize = as_u64(v); break;
+                case 's': o->fiberPerWarp = as_u64(v); break;
+                case 'h': o->MIfbTh = as_u64(v); break;
+                case 'g': o->gridSize = as_u64(v); break;
+                case 'v': o->verbose = std::atoi(v) == 1; break;
+                case 'c': o->correctness = std::atoi(v) == 1; break;
+                case 'd': o->factorDtype = std::atoi(v) == 1 ? MMCSF_BF16 : MMCSF_F32; break;
+                case 'i':
+                    MMCSF_REQUIRE(std::strlen(v) < sizeof(o->inFileName), MMCSF_ERR_ARG, "input path too long");
+                    std::strcpy(o->inFileName, v);
+                    break;
+                case 'o':
+                    MMCSF_REQUIRE(std::strlen(v) < sizeof(o->outFileName), MMCSF_ERR_ARG, "output path too long");
+                    std::strcpy(o->outFileName, v);
+                    break;
+                case 'p': std::snprintf(o->m0, sizeof(o->m0), "%s", v); break;
+                case 'q': std::snprintf(o->m1, sizeof(o->m1), "%s", v); break;
+                case 'r': std::snprintf(o->m2, sizeof(o->m2), "%s", v); break;
+                default: fail(MMCSF_ERR_ARG, std::string("unknown option: ") + argv[i - 1]);
+            }
+        }
+    });
+}
+
+int mmcsf_tensor_load_tns(const char *path, mmcsf_tensor **out) {
+    return guarded([&] {
+        MMCSF_REQUIRE(path && out, MMCSF_ERR_ARG, "null argument");
+        std::ifstream fp(path, std::ios::binary);
+        MMCSF_REQUIRE(fp.good(), MMCSF_ERR_IO, std::string(path) + " does not exist!");
+        std::string text((std::istreambuf_iterator<char>(fp)), std::istreambuf_iterator<char>());
+        const char *p = text.c_str();
+        char *end = nullptr;
+        auto next_u = [&](uint64_t &v) -> bool {
+            errno = 0;
+            unsigned long long x = std::strtoull(p, &end, 10);
+            if (end == p) return false;
+            v = x;
+            p = end;
+            return true;
+        };
+        uint64_t nd = 0;
+        MMCSF_REQUIRE(next_u(nd), MMCSF_ERR_IO, "cannot read the number of modes");
+        MMCSF_REQUIRE(nd == 3 || nd == 4, MMCSF_ERR_UNSUPPORTED, "Supported tensor dimension is 3 or 4.");
+        const int ndims = (int)nd;
+        uint64_t dims[kMaxDims];
+        for (int m = 0; m < ndims; ++m) MMCSF_REQUIRE(next_u(dims[m]), MMCSF_ERR_IO, "cannot read mode lengths");
+        std::vector<std::vector<uint32_t>> inds(ndims);
+        std::vector<float> vals;
+        for (;;) {
+            uint64_t c[kMaxDims];
+            if (!next_u(c[0])) break;
+            for (int m = 1; m < ndims; ++m) MMCSF_REQUIRE(next_u(c[m]), MMCSF_ERR_IO, "truncated nonzero line");
+            float v = std::strtof(p, &end);
+            MMCSF_REQUIRE(end != p, MMCSF_ERR_IO, "missing value");
+            p = end;
+            for (int m = 0; m < ndims; ++m) {
+                // 1-based -> 0-based (util.h:356-360)
+                MMCSF_REQUIRE(c[m] >= 1 && c[m] <= dims[m], MMCSF_ERR_IO, "coordinate out of range");
+                inds[m].push_back((uint32_t)(c[m] - 1));
+            }
+            vals.push_back(v);
+        }
+        check_shape(ndims, dims, vals.size());
+        auto h = std::make_unique<mmcsf_tensor>();
+        upload_tensor(ndims, dims, vals.size(), inds, vals.data(), h->t);
+        *out = h.release();
+    });
+}
+
+int mmcsf_tensor_from_host(int ndims, const uint64_t *dims, uint64_t nnz, const uint64_t *const *inds,
+                           const float *vals, mmcsf_tensor **out) {
+    return guarded([&] {
+        MMCSF_REQUIRE(dims && out && (nnz == 0 || (inds && vals)), MMCSF_ERR_ARG, "null argument");
+        check_shape(ndims, dims, nnz);
+        std::vector<std::vector<uint32_t>> h(ndims);
+        for (int m = 0; m < ndims; ++m) {
+            h[m].resize(nnz);
+            for (uint64_t x = 0; x < nnz; ++x) {
+                MMCSF_REQUIRE(inds[m][x] < dims[m], MMCSF_ERR_ARG, "coordinate out of range");
+                h[m][x] = (uint32_t)inds[m][x];
+            }
+        }
+        auto t = std::make_unique<mmcsf_tensor>();
+        upload_tensor(ndims, dims, nnz, h, vals, t->t);
+        *out = t.release();
+    });
+}
+
+int mmcsf_tensor_from_device(int ndims, const uint64_t *dims, uint64_t nnz, const uint32_t *const *inds,
+                             const float *vals, void *stream, mmcsf_tensor **out) {
+    return guarded([&] {
+        MMCSF_REQUIRE(dims && out && (nnz == 0 || (inds && vals)), MMCSF_ERR_ARG, "null argument");
+        check_shape(ndims, dims, nnz);
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        auto t = std::make_unique<mmcsf_tensor>();
+        t->t.ndims = ndims;
+        t->t.nnz = nnz;
+        DevArray<uint32_t> bad(1);
+        MMCSF_HIP(hipMemsetAsync(bad.p, 0, 4, s));
+        for (int m = 0; m < ndims; ++m) {
+            t->t.dims[m] = dims[m];
+            t->t.inds[m].alloc(nnz);
+            if (nnz) {
+                MMCSF_HIP(hipMemcpyAsync(t->t.inds[m].p, inds[m], nnz * 4, hipMemcpyDeviceToDevice, s));
+                hipLaunchKernelGGL(k_u32_check, dim3((unsigned)((nnz + 255) / 256)), dim3(256), 0, s,
+                                   t->t.inds[m].p, nnz, dims[m], bad.p);
+            }
+        }
+        t->t.vals.alloc(nnz);
+        if (nnz) MMCSF_HIP(hipMemcpyAsync(t->t.vals.p, vals, nnz * 4, hipMemcpyDeviceToDevice, s));
+        uint32_t hb = 0;
+        MMCSF_HIP(hipMemcpyAsync(&hb, bad.p, 4, hipMemcpyDeviceToHost, s));
+        MMCSF_HIP(hipStreamSynchronize(s));
+        MMCSF_REQUIRE(hb == 0, MMCSF_ERR_ARG, "coordinate out of range");
+        *out = t.release();
+    });
+}
+
+int mmcsf_tensor_generate(int ndims, const uint64_t *dims, uint64_t nnz, const int *law, const double *alpha,
+                          uint64_t seed, void *stream, mmcsf_tensor **out) {
+    return guarded([&] {
+        MMCSF_REQUIRE(dims && law && alpha && out, MMCSF_ERR_ARG, "null argument");
+        check_shape(ndims, dims, nnz);
+        long double space = 1;
+        for (int m = 0; m < ndims; ++m) space *= (long double)dims[m];
+        MMCSF_REQUIRE((long double)nnz <= space, MMCSF_ERR_ARG, "more nonzeros than coordinates");
+        auto t = std::make_unique<mmcsf_tensor>();
+        generate_tensor(ndims, dims, nnz, law, alpha, seed, t->t, static_cast<hipStream_t>(stream));
+        *out = t.release();
+    });
+}
+
+int mmcsf_tensor_info(const mmcsf_tensor *t, int *ndims, uint64_t *dims, uint64_t *nnz) {
+    return guarded([&] {
+        MMCSF_REQUIRE(t != nullptr, MMCSF_ERR_ARG, "null tensor");
+        if (ndims) *ndims = t->t.ndims;
+        if (dims)
+            for (int m = 0; m < t->t.ndims; ++m) dims[m] = t->t.dims[m];
+        if (nnz) *nnz = t->t.nnz;
+    });
+}
+
+int mmcsf_tensor_device_arrays(const mmcsf_tensor *t, const uint32_t **inds, const float **vals) {
+    return guarded([&] {
+        MMCSF_REQUIRE(t != nullptr, MMCSF_ERR_ARG, "null tensor");
+        if (inds)
+            for (int m = 0; m < t->t.ndims; ++m) inds[m] = t->t.inds[m].p;
+        if (vals) *vals = t->t.vals.p;
+    });
+}
+
+int mmcsf_tensor_copy_to_host(const mmcsf_tensor *t, uint64_t *const *inds, float *vals) {
+    return guarded([&] {
+        MMCSF_REQUIRE(t != nullptr, MMCSF_ERR_ARG, "null tensor");
+        for (int m = 0; m < t->t.ndims; ++m)
+            if (inds && inds[m]) {
+                auto h = to_host(t->t.inds[m], t->t.nnz);
+                for (uint64_t x = 0; x < t->t.nnz; ++x) inds[m][x] = h[x];
+            }
+        if (vals && t->t.nnz) MMCSF_HIP(hipMemcpy(vals, t->t.vals.p, t->t.nnz * 4, hipMemcpyDeviceToHost));
+    });
+}
+
+int mmcsf_tensor_copy_to_device(const mmcsf_tensor *t, uint32_t *const *inds, float *vals, void *stream) {
+    return guarded([&] {
+        MMCSF_REQUIRE(t != nullptr, MMCSF_ERR_ARG, "null tensor");
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        const uint64_t n = t->t.nnz;
+        for (int m = 0; m < t->t.ndims; ++m)
+            if (inds && inds[m] && n)
+                MMCSF_HIP(hipMemcpyAsync(inds[m], t->t.inds[m].p, n * 4, hipMemcpyDeviceToDevice, s));
+        if (vals && n) MMCSF_HIP(hipMemcpyAsync(vals, t->t.vals.p, n * 4, hipMemcpyDeviceToDevice, s));
+    });
+}
+
+void mmcsf_tensor_free(mmcsf_tensor *t) { delete t; }
+
+int mmcsf_format_build(const mmcsf_tensor *th, int type, const mmcsf_options *opt, void *stream,
+                       mmcsf_format **out) {
+    return guarded([&] {
+        MMCSF_REQUIRE(th && opt && out, MMCSF_ERR_ARG, "null argument");
+        MMCSF_REQUIRE(mmcsf_type_ok(type), MMCSF_ERR_UNSUPPORTED,
+                      "format type must be one of 1, 3, 8, 10, 12");
+        const Tensor &t = th->t;
+        // check_opt, util.h:245-248
+        MMCSF_REQUIRE(opt->mode < (uint64_t)t.ndims, MMCSF_ERR_ARG, "Mode cannot be larger than tensor dimension.");
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        auto fh = std::make_unique<mmcsf_format>();
+        Format &f = fh->f;
+        f.type = type;
+        f.ndims = t.ndims;
+        f.nnz = t.nnz;
+        for (int m = 0; m < t.ndims; ++m) f.dims[m] = t.dims[m];
+        f.redunMode = redundant_mode(t.ndims, t.dims);
+        f.fbrThreshold = opt->fbrThreashold;
+        const bool nat = opt->natOrdering != 0;
+        int order[kMaxDims];
+        DevArray<uint32_t> keys[kMaxDims];
+        DevArray<float> vals;
+        const uint64_t kNoSplit = 999999999ull;  // util.h:1323
+        if (type == MMCSF_TYPE_MMCSF) {
+            // every nonzero lands in partition 2 (util.h:2517), whose mode
+            // order is order_tensormode(X, 2) (init_tensor, util.h:372-387)
+            order_tensormode(t.ndims, t.dims, 2, nat, order);
+            sort_coo(t, order, keys, vals, s);
+            uint64_t fs = t.ndims == 4 ? 128 : kNoSplit;  // util.h:1325-1326
+            build_csf(t.ndims, t.dims, t.nnz, order, keys, std::move(vals), opt->fbrThreashold, fs, f.csf, s);
+        } else {
+            order_tensormode(t.ndims, t.dims, (int)opt->mode, nat, order);
+            if (type == MMCSF_TYPE_COO || type == MMCSF_TYPE_COO_CPU) {
+                for (int l = 0; l < t.ndims; ++l) f.cooOrder[l] = order[l];
+                sort_coo(t, order, f.cooKeys, f.cooVals, s);
+            } else {
+                sort_coo(t, order, keys, vals, s);
+                build_csf(t.ndims, t.dims, t.nnz, order, keys, std::move(vals), opt->fbrThreashold, kNoSplit,
+                          f.csf, s);
+                if (type == MMCSF_TYPE_BCSF) {
+                    build_bins_tiled(f.csf, s);
+                } else {
+                    build_hyb(t.ndims, t.dims, std::move(f.csf), f.hyb, s);
+                }
+            }
+        }
+        *out = fh.release();
+    });
+}
+
+int mmcsf_format_info_get(const mmcsf_format *fh, mmcsf_format_info *info) {
+    return guarded([&] {
+        MMCSF_REQUIRE(fh && info, MMCSF_ERR_ARG, "null argument");
+        const Format &f = fh->f;
+        std::memset(info, 0, sizeof(*info));
+        info->type = f.type;
+        info->ndims = f.ndims;
+        info->totNnz = f.nnz;
+        info->redunMode = f.redunMode;
+        for (int m = 0; m < f.ndims; ++m) info->dims[m] = f.dims[m];
+        const Csf *c = nullptr;
+        if (f.type == MMCSF_TYPE_COO || f.type == MMCSF_TYPE_COO_CPU) {
+            for (int l = 0; l < f.ndims; ++l) info->modeOrder[l] = f.cooOrder[l];
+        } else if (f.type == MMCSF_TYPE_HBCSF) {
+            c = &f.hyb.csf;
+            for (int l = 0; l < f.ndims; ++l) info->modeOrder[l] = f.hyb.order[l];
+            info->COOnnz = f.hyb.cooNnz;
+            info->CSLnnz = f.hyb.cslNnz;
+            info->HCSRnnz = f.hyb.csf.nnz;
+        } else {
+            c = &f.csf;
+        }
+        if (c) {
+            for (int l = 0; l < f.ndims; ++l) {
+                info->modeOrder[l] = c->order[l];
+                info->nLevel[l] = c->cnt[l];
+            }
+            // TiledTensor::nFibers = fbrPtr[1].size() (util.h:1424); HYB: the
+            // leaf-fiber pointer count (util.h:980, 1129)
+            info->nFibers = (f.type == MMCSF_TYPE_HBCSF ? c->cnt[f.ndims - 2] : c->cnt[1]) + 1;
+            info->droppedSlices = c->dropped;
+        }
+    });
+}
+
+int mmcsf_format_export(const mmcsf_format *fh, const char *name, int k, void *dst, uint64_t cap, uint64_t *len) {
+    return guarded([&] {
+        MMCSF_REQUIRE(fh && name && len, MMCSF_ERR_ARG, "null argument");
+        const Format &f = fh->f;
+        const int N = f.ndims;
+        std::string nm(name);
+        std::vector<uint64_t> u;
+        std::vector<float> fl;
+        bool isFloat = false;
+        const bool hyb = f.type == MMCSF_TYPE_HBCSF;
+        const bool coo = f.type == MMCSF_TYPE_COO || f.type == MMCSF_TYPE_COO_CPU;
+        const Csf &c = hyb ? f.hyb.csf : f.csf;
+        auto widen = [&](const std::vector<uint32_t> &v) { u.assign(v.begin(), v.end()); };
+        auto need_level = [&](int lvl) {
+            MMCSF_REQUIRE(!coo && lvl >= 0 && lvl < N - 1, MMCSF_ERR_ARG, "bad level for " + nm);
+        };
+        if (nm == "fbrPtr") {
+            need_level(k);
+            widen(to_host(c.ptr[k], c.cnt[k] + 1));
+        } else if (nm == "fbrIdx") {
+            need_level(k);
+            widen(to_host(c.idx[k], c.cnt[k]));
+            // create_TiledHCSR appends a trailing leaf-fiber index
+            // (util.h:1418-1419); create_HYB does not
+            if (!hyb && k == N - 2 && c.cnt[k]) u.push_back(u.back());
+        } else if (nm == "inds") {
+            MMCSF_REQUIRE(k >= 0 && k < N, MMCSF_ERR_ARG, "bad mode for inds");
+            if (coo) {
+                int l = level_of(f.cooOrder, N, k);
+                widen(to_host(f.cooKeys[l], f.nnz));
+            } else {
+                widen(expand_level(c, level_of(c.order, N, k)));
+            }
+        } else if (nm == "vals") {
+            isFloat = true;
+            fl = coo ? to_host(f.cooVals, f.nnz) : to_host(c.vals, c.nnz);
+        } else if (nm == "fbrLikeSlcInds") {
+            MMCSF_REQUIRE(!coo && !hyb, MMCSF_ERR_ARG, "fbrLikeSlcInds exists for B-CSF and MM-CSF");
+            widen(to_host(c.fbrSlc, c.cnt[1]));
+            // nFibers entries; the last one is never written by the reference
+            u.push_back(u.empty() ? 0 : u.back());
+        } else if (nm == "slcMapperBin") {
+            MMCSF_REQUIRE(c.hasBins && k >= 0 && k < kNumBins, MMCSF_ERR_ARG, "no such bin");
+            uint64_t off = 0;
+            for (int b = 0; b < k; ++b) off += c.binCount[b];
+            std::vector<uint32_t> all = to_host(c.binList, off + c.binCount[k]);
+            u.assign(all.begin() + off, all.end());
+        } else if (nm == "partPerNnz") {
+            MMCSF_REQUIRE(f.type == MMCSF_TYPE_MMCSF, MMCSF_ERR_ARG, "partPerNnz exists for MM-CSF");
+            u.assign(f.nnz, 2);
+        } else if (hyb && nm == "COOinds") {
+            MMCSF_REQUIRE(k >= 0 && k < N, MMCSF_ERR_ARG, "bad mode");
+            widen(to_host(f.hyb.cooInd[level_of(f.hyb.order, N, k)], f.hyb.cooNnz));
+        } else if (hyb && nm == "COOvals") {
+            isFloat = true;
+            fl = to_host(f.hyb.cooVal, f.hyb.cooNnz);
+        } else if (hyb && nm == "CSLslicePtr") {
+            widen(to_host(f.hyb.cslPtr, f.hyb.nCsl + 1));
+        } else if (hyb && nm == "CSLsliceIdx") {
+            widen(to_host(f.hyb.cslIdx, f.hyb.nCsl));
+        } else if (hyb && nm == "CSLinds") {
+            int l = level_of(f.hyb.order, N, k);
+            MMCSF_REQUIRE(N == 3 && (l == 1 || l == 2), MMCSF_ERR_ARG, "CSLinds exists for modeOrder[1..2] of 3-D");
+            widen(to_host(l == 1 ? f.hyb.cslInd1 : f.hyb.cslInd2, f.hyb.cslNnz));
+        } else if (hyb && nm == "CSLvals") {
+            isFloat = true;
+            fl = to_host(f.hyb.cslVal, f.hyb.cslNnz);
+        } else if (hyb && nm == "CSLslcMapperBin") {
+            MMCSF_REQUIRE(k >= 0 && k < kNumBins, MMCSF_ERR_ARG, "no such bin");
+            uint64_t off = 0;
+            for (int b = 0; b < k; ++b) off += f.hyb.cslBinCount[b];
+            std::vector<uint32_t> all = to_host(f.hyb.cslBinList, off + f.hyb.cslBinCount[k]);
+            u.assign(all.begin() + off, all.end());
+        } else {
+            fail(MMCSF_ERR_ARG, "unknown field " + nm);
+        }
+        *len = isFloat ? fl.size() : u.size();
+        if (dst && cap >= *len) {
+            if (isFloat) std::memcpy(dst, fl.data(), fl.size() * 4);
+            else std::memcpy(dst, u.data(), u.size() * 8);
+        }
+    });
+}
+
+void mmcsf_format_free(mmcsf_format *f) { delete f; }
+
+int mmcsf_mttkrp(const mmcsf_format *fh, int mode, int R, int dtype, const void *const *factors, float *out,
+                 void *stream) {
+    return guarded([&] {
+        MMCSF_REQUIRE(fh && factors && out, MMCSF_ERR_ARG, "null argument");
+        MMCSF_REQUIRE(dtype == MMCSF_F32 || dtype == MMCSF_BF16, MMCSF_ERR_ARG, "dtype must be F32 or BF16");
+        const Format &f = fh->f;
+        MMCSF_REQUIRE(mode >= 0 && mode < f.ndims, MMCSF_ERR_ARG, "Mode cannot be larger than tensor dimension.");
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        Factors F{};
+        for (int m = 0; m < f.ndims; ++m) F.U[m] = factors[m];
+        F.dtype = dtype;
+        F.R = R;
+        switch (f.type) {
+            case MMCSF_TYPE_COO:
+            case MMCSF_TYPE_COO_CPU:
+                MMCSF_REQUIRE(mode == f.cooOrder[0], MMCSF_ERR_ARG, "COO format serves its root mode only");
+                mttkrp_coo(f, F, out, s);
+                break;
+            case MMCSF_TYPE_BCSF:
+                MMCSF_REQUIRE(mode == f.csf.order[0], MMCSF_ERR_ARG, "B-CSF serves its root mode only");
+                mttkrp_csf(f.csf, f.dims, 0, F, out, s);
+                break;
+            case MMCSF_TYPE_HBCSF:
+                MMCSF_REQUIRE(mode == f.hyb.order[0], MMCSF_ERR_ARG, "HB-CSF serves its root mode only");
+                mttkrp_hyb(f.hyb, f.dims, F, out, s);
+                break;
+            default:
+                mttkrp_csf(f.csf, f.dims, level_of(f.csf.order, f.ndims, mode), F, out, s);
+                break;
+        }
+    });
+}
+
+int mmcsf_mttkrp_allmode(const mmcsf_format *fh, int R, int dtype, const void *const *factors, float *const *outs,
+                         void *stream, float *ms_per_mode) {
+    return guarded([&] {
+        MMCSF_REQUIRE(fh && factors && outs, MMCSF_ERR_ARG, "null argument");
+        const Format &f = fh->f;
+        MMCSF_REQUIRE(f.type == MMCSF_TYPE_MMCSF, MMCSF_ERR_ARG, "all-mode MTTKRP needs an MM-CSF format");
+        MMCSF_REQUIRE(dtype == MMCSF_F32 || dtype == MMCSF_BF16, MMCSF_ERR_ARG, "dtype must be F32 or BF16");
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        // one event pair per mode, around the MTTKRP kernel only
+        hipEvent_t ev[2 * kMaxDims] = {};
+        struct Events {
+            hipEvent_t *e;
+            int n;
+            ~Events() {
+                for (int i = 0; i < n; ++i)
+                    if (e[i]) (void)hipEventDestroy(e[i]);
+            }
+        } guard{ev, 2 * kMaxDims};
+        if (ms_per_mode)
+            for (int i = 0; i < 2 * f.ndims; ++i) MMCSF_HIP(hipEventCreate(&ev[i]));
+        Factors F{};
+        for (int m = 0; m < f.ndims; ++m) F.U[m] = factors[m];
+        F.dtype = dtype;
+        F.R = R;
+        for (int m = 0; m < f.ndims; ++m) {
+            MMCSF_REQUIRE(outs[m] != nullptr, MMCSF_ERR_ARG, "missing output");
+            mttkrp_csf(f.csf, f.dims, level_of(f.csf.order, f.ndims, m), F, outs[m], s,
+                       ms_per_mode ? ev[2 * m] : nullptr, ms_per_mode ? ev[2 * m + 1] : nullptr);
+        }
+        if (ms_per_mode) {
+            MMCSF_HIP(hipEventSynchronize(ev[2 * f.ndims - 1]));
+            for (int m = 0; m < f.ndims; ++m)
+                MMCSF_HIP(hipEventElapsedTime(&ms_per_mode[m], ev[2 * m], ev[2 * m + 1]));
+        }
+    });
+}
+
+int mmcsf_mttkrp_coo_host(const mmcsf_format *fh, int mode, int R, const float *const *U, float *out) {
+    return guarded([&] {
+        MMCSF_REQUIRE(fh && U && out && R >= 1, MMCSF_ERR_ARG, "bad argument");
+        const Format &f = fh->f;
+        MMCSF_REQUIRE(f.type == MMCSF_TYPE_COO_CPU || f.type == MMCSF_TYPE_COO, MMCSF_ERR_ARG,
+                      "the host COO MTTKRP needs a COO format");
+        MMCSF_REQUIRE(mode >= 0 && mode < f.ndims, MMCSF_ERR_ARG, "Mode cannot be larger than tensor dimension.");
+        const int N = f.ndims;
+        std::vector<std::vector<uint32_t>> ind(N);
+        for (int m = 0; m < N; ++m) ind[m] = to_host(f.cooKeys[level_of(f.cooOrder, N, m)], f.nnz);
+        std::vector<float> v = to_host(f.cooVals, f.nnz);
+        // MTTKRP_COO_CPU[_4D] (mttkrp_cpu.cpp:60-113): cyclic roles, stored
+        // order, sequential float accumulation into out
+        int cm[kMaxDims];
+        for (int m = 0; m < N; ++m) cm[m] = (m + mode) % N;
+        const size_t r = (size_t)R;
+        for (uint64_t x = 0; x < f.nnz; ++x) {
+            float *o = out + ind[cm[0]][x] * r;
+            const float *a = U[cm[1]] + ind[cm[1]][x] * r;
+            const float *b = U[cm[2]] + ind[cm[2]][x] * r;
+            if (N == 3) {
+                for (size_t q = 0; q < r; ++q) {
+                    float tmp = v[x] * a[q] * b[q];
+                    o[q] += tmp;
+                }
+            } else {
+                const float *c = U[cm[3]] + ind[cm[3]][x] * r;
+                for (size_t q = 0; q < r; ++q) {
+                    float tmp = v[x] * a[q] * b[q] * c[q];
+                    o[q] += tmp;
+                }
+            }
+        }
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,202 @@
+// Internal declarations shared by the library's translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "mmcsf.h"
+
+namespace mmcsf {
+
+// ---------------------------------------------------------------------------
+// errors: internal code throws mmcsf::Error; the C ABI catches it, stores the
+// message for mmcsf_last_error() and returns the status code.
+// ---------------------------------------------------------------------------
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+void set_last_error(const std::string &msg);
+
+[[noreturn]] inline void fail(int code, const std::string &msg) { throw Error(code, msg); }
+
+#define MMCSF_HIP(call)                                                              \
+    do {                                                                             \
+        hipError_t e_ = (call);                                                      \
+        if (e_ != hipSuccess)                                                        \
+            ::mmcsf::fail(e_ == hipErrorOutOfMemory ? MMCSF_ERR_NOMEM : MMCSF_ERR_HIP, \
+                          std::string(#call) + ": " + hipGetErrorString(e_) + " at " + \
+                              __FILE__ + ":" + std::to_string(__LINE__));            \
+    } while (0)
+
+#define MMCSF_REQUIRE(cond, code, msg)                                               \
+    do {                                                                             \
+        if (!(cond)) ::mmcsf::fail((code), (msg));                                   \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// owning device buffer
+// ---------------------------------------------------------------------------
+template <typename T>
+struct DevArray {
+    T *p = nullptr;
+    size_t n = 0;
+    DevArray() = default;
+    explicit DevArray(size_t count) { alloc(count); }
+    DevArray(const DevArray &) = delete;
+    DevArray &operator=(const DevArray &) = delete;
+    DevArray(DevArray &&o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+    DevArray &operator=(DevArray &&o) noexcept {
+        if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+        return *this;
+    }
+    ~DevArray() { release(); }
+    void alloc(size_t count) {
+        release();
+        n = count;
+        if (count) MMCSF_HIP(hipMalloc(reinterpret_cast<void **>(&p), count * sizeof(T) + 16));
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    T *get() const { return p; }
+};
+
+// ---------------------------------------------------------------------------
+// tensors and formats (device resident)
+// ---------------------------------------------------------------------------
+constexpr int kMaxDims = 4;
+constexpr int kNumBins = 10;             // Options::nBin, util.h:205
+constexpr uint32_t kHotBit = 0x80000000u;  // target-array tag of a privatised row
+
+// COO tensor: class Tensor (util.h:80-108) without the derived members.
+struct Tensor {
+    int ndims = 0;
+    uint64_t dims[kMaxDims] = {0, 0, 0, 0};
+    uint64_t nnz = 0;
+    DevArray<uint32_t> inds[kMaxDims];   // per tensor mode
+    DevArray<float> vals;
+};
+
+// Top-k rows of a scatter output, privatised in LDS by the scatter kernels.
+struct HotTable {
+    uint32_t count = 0;                  // rows in the table (<= kMaxHot)
+    DevArray<uint32_t> rows;             // rank -> row
+    DevArray<uint32_t> tgt;              // per contribution: row, or kHotBit|rank
+};
+
+// One compressed-sparse-fiber tree with N-1 index levels (TiledTensor,
+// util.h:110-139, for one tile / one partition).  Level l = 0..N-2 has cnt[l]
+// entries; ptr[l] has cnt[l]+1 entries pointing into level l+1 (into the
+// nonzeros for the leaf-fiber level N-2).
+struct Csf {
+    int ndims = 0;
+    int order[kMaxDims] = {0, 1, 2, 3};  // modeOrder
+    uint64_t nnz = 0;
+    uint64_t cnt[kMaxDims] = {0, 0, 0, 0};
+    DevArray<uint32_t> ptr[kMaxDims - 1];
+    DevArray<uint32_t> idx[kMaxDims - 1];
+    DevArray<uint32_t> leaf;             // coordinate of order[N-1] per nonzero
+    DevArray<float> vals;
+    DevArray<uint32_t> fbrSlc;           // fbrLikeSlcInds: slice coord per level-1 entry
+    // slice bins (make_TiledBin / make_HybBin)
+    bool hasBins = false;
+    DevArray<uint32_t> binList;          // concatenated slcMapperBin[0..9]
+    uint64_t binCount[kNumBins] = {0};
+    uint64_t dropped = 0;
+    // privatisation tables for the scatter roles, indexed by output level
+    HotTable hot[kMaxDims];
+    bool hotBuilt[kMaxDims] = {false, false, false, false};
+};
+
+// HB-CSF parts (HYBTensor, util.h:141-173)
+struct Hyb {
+    int ndims = 0;
+    int order[kMaxDims] = {0, 1, 2, 3};
+    uint64_t cooNnz = 0, cslNnz = 0, nCsl = 0;
+    DevArray<uint32_t> cooInd[kMaxDims];  // by level
+    DevArray<float> cooVal;
+    DevArray<uint32_t> cslPtr, cslIdx, cslInd1, cslInd2;
+    DevArray<float> cslVal;
+    DevArray<uint32_t> cslBinList;
+    uint64_t cslBinCount[kNumBins] = {0};
+    Csf csf;                               // CSF part (hasBins = true)
+};
+
+struct Format {
+    int type = 0;
+    int ndims = 0;
+    uint64_t dims[kMaxDims] = {0, 0, 0, 0};
+    uint64_t nnz = 0;
+    int redunMode = -1;
+    uint64_t fbrThreshold = 0;
+    // type 3: sorted COO (levels in modeOrder order)
+    int cooOrder[kMaxDims] = {0, 1, 2, 3};
+    DevArray<uint32_t> cooKeys[kMaxDims];
+    DevArray<float> cooVals;
+    // types 8 / 12
+    Csf csf;
+    // type 10
+    Hyb hyb;
+};
+
+// ---------------------------------------------------------------------------
+// builder (builder.hip)
+// ---------------------------------------------------------------------------
+void order_tensormode(int ndims, const uint64_t *dims, int mode, bool nat, int *order);
+
+// Sorts the tensor's nonzeros by the full key in `order`; outputs level keys
+// keys[l] = coordinate of order[l], and permuted values.
+void sort_coo(const Tensor &t, const int *order, DevArray<uint32_t> *keys, DevArray<float> &vals,
+              hipStream_t s);
+
+// CSF scan with the -f split and (4-D) fbrS split; consumes sorted keys.
+void build_csf(int ndims, const uint64_t *dims, uint64_t nnz, const int *order,
+               DevArray<uint32_t> *keys, DevArray<float> &&vals, uint64_t fbrThreshold,
+               uint64_t fbrSThreshold, Csf &out, hipStream_t s);
+
+void build_bins_tiled(Csf &c, hipStream_t s);           // make_TiledBin
+void build_hyb(int ndims, const uint64_t *dims, Csf &&src, Hyb &out, hipStream_t s);
+void build_hot_table(const Csf &c, int level, uint64_t rows, HotTable &h, hipStream_t s);
+
+// ---------------------------------------------------------------------------
+// MTTKRP launchers (mttkrp.hip)
+// ---------------------------------------------------------------------------
+struct Factors {
+    const void *U[kMaxDims];
+    int dtype;
+    int R;
+};
+
+// evStart / evEnd (optional) bracket the MTTKRP kernel only, after the
+// output has been zeroed.
+void mttkrp_csf(const Csf &c, const uint64_t *dims, int outLevel, const Factors &F, float *out,
+                hipStream_t s, hipEvent_t evStart = nullptr, hipEvent_t evEnd = nullptr);
+void mttkrp_coo(const Format &f, const Factors &F, float *out, hipStream_t s);
+void mttkrp_hyb(const Hyb &h, const uint64_t *dims, const Factors &F, float *out, hipStream_t s);
+void fill_zero(float *p, uint64_t n, hipStream_t s);
+
+// generator (generate.hip)
+void generate_tensor(int ndims, const uint64_t *dims, uint64_t nnz, const int *law,
+                     const double *alpha, uint64_t seed, Tensor &t, hipStream_t s);
+
+int device_cu_count();
+
+}  // namespace mmcsf
+
+// C-ABI handle types wrap the internal structures.
+struct mmcsf_tensor {
+    mmcsf::Tensor t;
+};
+struct mmcsf_format {
+    mmcsf::Format f;
+};

@@ -1,0 +1,318 @@
+"""Python binding of the gfx950 sparse-MTTKRP library (ctypes over include/mmcsf.h).
+
+The library ``libmmcsf.so`` sits next to this package's parent directory
+(``mm-csf_amd/libmmcsf.so``) and is built by ``mm-csf_amd/Makefile``.  Loading
+fails loudly when it is missing: there is no CPU fallback for any GPU entry
+point.  PyTorch is used only for device memory and streams.
+
+Mirrors the reference driver's objects (src/util.h): ``Tensor`` (class Tensor,
+util.h:80-108), ``Format`` (TiledTensor / HYBTensor, util.h:110-173),
+``Options`` (util.h:189-236) and the MTTKRP launchers of src/mttkrp_gpu.h.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+__all__ = [
+    "lib", "MMCSFError", "Options", "Tensor", "Format", "mttkrp", "mttkrp_allmode",
+    "TYPE_COO_CPU", "TYPE_COO", "TYPE_BCSF", "TYPE_HBCSF", "TYPE_MMCSF", "F32", "BF16",
+    "parse_options", "to_bf16",
+]
+
+TYPE_COO_CPU, TYPE_COO, TYPE_BCSF, TYPE_HBCSF, TYPE_MMCSF = 1, 3, 8, 10, 12
+F32, BF16 = 0, 1
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "libmmcsf.so")
+
+
+class MMCSFError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"mmcsf error {code}: {msg}")
+        self.code = code
+
+
+class Options(C.Structure):
+    _fields_ = [
+        ("R", C.c_uint64), ("mode", C.c_uint64), ("impType", C.c_uint64),
+        ("warpPerSlice", C.c_uint64), ("nTile", C.c_uint64), ("gridSize", C.c_uint64),
+        ("TBsize", C.c_uint64), ("MIfbTh", C.c_uint64), ("fiberPerWarp", C.c_uint64),
+        ("verbose", C.c_int), ("correctness", C.c_int), ("nBin", C.c_uint64),
+        ("cpdIters", C.c_uint64), ("natOrdering", C.c_int), ("fbrThreashold", C.c_uint64),
+        ("factorDtype", C.c_int), ("inFileName", C.c_char * 4096), ("outFileName", C.c_char * 4096),
+        ("m0", C.c_char * 16), ("m1", C.c_char * 16), ("m2", C.c_char * 16),
+    ]
+
+
+class FormatInfo(C.Structure):
+    _fields_ = [
+        ("type", C.c_int), ("ndims", C.c_int), ("dims", C.c_uint64 * 4), ("totNnz", C.c_uint64),
+        ("modeOrder", C.c_int * 4), ("nLevel", C.c_uint64 * 4), ("nFibers", C.c_uint64),
+        ("droppedSlices", C.c_uint64), ("COOnnz", C.c_uint64), ("CSLnnz", C.c_uint64),
+        ("HCSRnnz", C.c_uint64), ("redunMode", C.c_int),
+    ]
+
+
+_LIB = None
+VP = C.c_void_p
+U64P = C.POINTER(C.c_uint64)
+
+
+def lib():
+    """The loaded libmmcsf.so (raises when it has not been built)."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"libmmcsf.so not built: {LIB_PATH} (run make -C mm-csf_amd)")
+    L = C.CDLL(LIB_PATH)
+    L.mmcsf_last_error.restype = C.c_char_p
+    L.mmcsf_version.restype = C.c_char_p
+    L.mmcsf_options_default.argtypes = [C.POINTER(Options)]
+    L.mmcsf_options_parse.argtypes = [C.c_int, C.POINTER(C.c_char_p), C.POINTER(Options)]
+    L.mmcsf_tensor_load_tns.argtypes = [C.c_char_p, C.POINTER(VP)]
+    L.mmcsf_tensor_from_host.argtypes = [C.c_int, U64P, C.c_uint64, C.POINTER(U64P),
+                                         C.POINTER(C.c_float), C.POINTER(VP)]
+    L.mmcsf_tensor_from_device.argtypes = [C.c_int, U64P, C.c_uint64, C.POINTER(VP), VP, VP,
+                                           C.POINTER(VP)]
+    L.mmcsf_tensor_generate.argtypes = [C.c_int, U64P, C.c_uint64, C.POINTER(C.c_int),
+                                        C.POINTER(C.c_double), C.c_uint64, VP, C.POINTER(VP)]
+    L.mmcsf_tensor_info.argtypes = [VP, C.POINTER(C.c_int), U64P, U64P]
+    L.mmcsf_tensor_device_arrays.argtypes = [VP, C.POINTER(VP), C.POINTER(VP)]
+    L.mmcsf_tensor_copy_to_host.argtypes = [VP, C.POINTER(U64P), C.POINTER(C.c_float)]
+    L.mmcsf_tensor_copy_to_device.argtypes = [VP, C.POINTER(VP), VP, VP]
+    L.mmcsf_tensor_free.argtypes = [VP]
+    L.mmcsf_tensor_free.restype = None
+    L.mmcsf_format_build.argtypes = [VP, C.c_int, C.POINTER(Options), VP, C.POINTER(VP)]
+    L.mmcsf_format_info_get.argtypes = [VP, C.POINTER(FormatInfo)]
+    L.mmcsf_format_export.argtypes = [VP, C.c_char_p, C.c_int, VP, C.c_uint64, U64P]
+    L.mmcsf_format_free.argtypes = [VP]
+    L.mmcsf_format_free.restype = None
+    L.mmcsf_mttkrp.argtypes = [VP, C.c_int, C.c_int, C.c_int, C.POINTER(VP), VP, VP]
+    L.mmcsf_mttkrp_allmode.argtypes = [VP, C.c_int, C.c_int, C.POINTER(VP), C.POINTER(VP), VP,
+                                       C.POINTER(C.c_float)]
+    L.mmcsf_mttkrp_coo_host.argtypes = [VP, C.c_int, C.c_int, C.POINTER(C.POINTER(C.c_float)),
+                                        C.POINTER(C.c_float)]
+    _LIB = L
+    return L
+
+
+def _check(rc):
+    if rc != 0:
+        raise MMCSFError(rc, lib().mmcsf_last_error().decode(errors="replace"))
+
+
+def parse_options(argv):
+    """parse_cmd_options (util.h:3341-3442) -> Options."""
+    o = Options()
+    args = [b"mttkrp"] + [a.encode() if isinstance(a, str) else a for a in argv]
+    arr = (C.c_char_p * len(args))(*args)
+    _check(lib().mmcsf_options_parse(len(args), arr, C.byref(o)))
+    return o
+
+
+def default_options():
+    o = Options()
+    _check(lib().mmcsf_options_default(C.byref(o)))
+    return o
+
+
+def _stream_ptr(stream=None):
+    import torch
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return VP(s.cuda_stream)
+
+
+class Tensor:
+    """Device COO tensor (class Tensor, util.h:80-108)."""
+
+    def __init__(self, handle):
+        self._h = VP(handle) if not isinstance(handle, VP) else handle
+
+    @classmethod
+    def from_numpy(cls, dims, inds, vals):
+        dims = np.ascontiguousarray(dims, dtype=np.uint64)
+        inds = [np.ascontiguousarray(a, dtype=np.uint64) for a in inds]
+        vals = np.ascontiguousarray(vals, dtype=np.float32)
+        nnz = len(vals)
+        ptrs = (U64P * len(inds))(*[a.ctypes.data_as(U64P) for a in inds])
+        h = VP()
+        _check(lib().mmcsf_tensor_from_host(len(dims), dims.ctypes.data_as(U64P), nnz, ptrs,
+                                            vals.ctypes.data_as(C.POINTER(C.c_float)), C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def load_tns(cls, path):
+        h = VP()
+        _check(lib().mmcsf_tensor_load_tns(str(path).encode(), C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def generate(cls, dims, nnz, law, alpha, seed, stream=None):
+        dims = np.ascontiguousarray(dims, dtype=np.uint64)
+        n = len(dims)
+        lw = (C.c_int * n)(*[int(x) for x in law])
+        al = (C.c_double * n)(*[float(x) for x in alpha])
+        h = VP()
+        _check(lib().mmcsf_tensor_generate(n, dims.ctypes.data_as(U64P), int(nnz), lw, al, int(seed),
+                                           _stream_ptr(stream), C.byref(h)))
+        return cls(h)
+
+    def info(self):
+        nd = C.c_int()
+        dims = (C.c_uint64 * 4)()
+        nnz = C.c_uint64()
+        _check(lib().mmcsf_tensor_info(self._h, C.byref(nd), dims, C.byref(nnz)))
+        return nd.value, [int(dims[i]) for i in range(nd.value)], nnz.value
+
+    def to_numpy(self):
+        nd, dims, nnz = self.info()
+        inds = [np.zeros(nnz, dtype=np.uint64) for _ in range(nd)]
+        vals = np.zeros(nnz, dtype=np.float32)
+        ptrs = (U64P * nd)(*[a.ctypes.data_as(U64P) for a in inds])
+        _check(lib().mmcsf_tensor_copy_to_host(self._h, ptrs, vals.ctypes.data_as(C.POINTER(C.c_float))))
+        return dims, [a.astype(np.int64) for a in inds], vals
+
+    def to_torch(self, device=None, stream=None):
+        """(inds, vals) as device torch tensors: int32 coordinates per mode, float32 values."""
+        import torch
+        nd, dims, nnz = self.info()
+        dev = device if device is not None else torch.device("cuda", torch.cuda.current_device())
+        inds = [torch.empty(nnz, dtype=torch.int32, device=dev) for _ in range(nd)]
+        vals = torch.empty(nnz, dtype=torch.float32, device=dev)
+        ip = (VP * nd)(*[VP(a.data_ptr()) for a in inds])
+        _check(lib().mmcsf_tensor_copy_to_device(self._h, ip, VP(vals.data_ptr()), _stream_ptr(stream)))
+        return inds, vals
+
+    def device_arrays(self):
+        nd, _, _ = self.info()
+        ip = (VP * 4)()
+        vp = VP()
+        _check(lib().mmcsf_tensor_device_arrays(self._h, ip, C.byref(vp)))
+        return [ip[i] for i in range(nd)], vp.value
+
+    def free(self):
+        if self._h is not None and self._h.value:
+            lib().mmcsf_tensor_free(self._h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Format:
+    """Device format of one driver type (-t 1/3/8/10/12)."""
+
+    def __init__(self, tensor, fmt_type, mode=0, fbr_threshold=99999999, stream=None, opts=None):
+        o = opts if opts is not None else default_options()
+        o.mode = int(mode)
+        o.fbrThreashold = int(fbr_threshold) & 0xFFFFFFFFFFFFFFFF
+        h = VP()
+        _check(lib().mmcsf_format_build(tensor._h, int(fmt_type), C.byref(o), _stream_ptr(stream),
+                                        C.byref(h)))
+        self._h = h
+        self.type = fmt_type
+
+    def info(self):
+        i = FormatInfo()
+        _check(lib().mmcsf_format_info_get(self._h, C.byref(i)))
+        return {
+            "type": i.type, "ndims": i.ndims, "dims": list(i.dims[: i.ndims]), "totNnz": i.totNnz,
+            "modeOrder": list(i.modeOrder[: i.ndims]), "nLevel": list(i.nLevel[: i.ndims]),
+            "nFibers": i.nFibers, "droppedSlices": i.droppedSlices, "COOnnz": i.COOnnz,
+            "CSLnnz": i.CSLnnz, "HCSRnnz": i.HCSRnnz, "redunMode": i.redunMode,
+        }
+
+    def export(self, name, k=0):
+        """A reference member as a numpy array (uint64, or float32 for values)."""
+        n = C.c_uint64()
+        _check(lib().mmcsf_format_export(self._h, name.encode(), int(k), None, 0, C.byref(n)))
+        is_float = name in ("vals", "COOvals", "CSLvals")
+        out = np.zeros(max(n.value, 1), dtype=np.float32 if is_float else np.uint64)
+        _check(lib().mmcsf_format_export(self._h, name.encode(), int(k), out.ctypes.data_as(VP),
+                                         n.value, C.byref(n)))
+        return out[: n.value]
+
+    def free(self):
+        if self._h is not None and self._h.value:
+            lib().mmcsf_format_free(self._h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def to_bf16(t):
+    """float32 torch tensor -> bfloat16 bit patterns (round to nearest even)."""
+    import torch
+    return t.to(torch.bfloat16).contiguous()
+
+
+def _factor_ptrs(factors):
+    return (VP * len(factors))(*[VP(f.data_ptr()) if f is not None else VP(0) for f in factors])
+
+
+def _dtype_of(factors):
+    import torch
+    dts = {f.dtype for f in factors if f is not None}
+    if dts == {torch.float32}:
+        return F32
+    if dts == {torch.bfloat16}:
+        return BF16
+    raise ValueError("factors must all be float32 or all bfloat16")
+
+
+def _check_factors(dims, factors, R, skip=None):
+    if len(factors) != len(dims):
+        raise ValueError(f"expected {len(dims)} factor matrices, got {len(factors)}")
+    for m, f in enumerate(factors):
+        if m == skip and f is None:
+            continue
+        if f is None or not f.is_cuda or not f.is_contiguous() or tuple(f.shape) != (dims[m], R):
+            raise ValueError(f"factor {m} must be a contiguous device tensor of shape ({dims[m]}, {R})")
+
+
+def mttkrp(fmt, mode, factors, out=None, stream=None):
+    """MTTKRP of ``mode``; factors are device torch tensors (float32 or bfloat16)."""
+    import torch
+    R = next(f for f in factors if f is not None).shape[1]
+    dims = fmt.info()["dims"]
+    if not 0 <= int(mode) < len(dims):
+        raise MMCSFError(-1, "Mode cannot be larger than tensor dimension.")
+    rows = dims[mode]
+    _check_factors(dims, factors, R, skip=mode)
+    if out is None:
+        dev = next(f for f in factors if f is not None).device
+        out = torch.empty((rows, R), dtype=torch.float32, device=dev)
+    if tuple(out.shape) != (rows, R) or out.dtype != torch.float32 or not out.is_contiguous():
+        raise ValueError(f"out must be a contiguous float32 tensor of shape ({rows}, {R})")
+    _check(lib().mmcsf_mttkrp(fmt._h, int(mode), int(R), _dtype_of(factors), _factor_ptrs(factors),
+                              VP(out.data_ptr()), _stream_ptr(stream)))
+    return out
+
+
+def mttkrp_allmode(fmt, factors, outs=None, stream=None, timed=False):
+    """All-mode MM-CSF MTTKRP; returns (outs, ms_per_mode or None)."""
+    import torch
+    R = factors[0].shape[1]
+    info = fmt.info()
+    _check_factors(info["dims"], factors, R)
+    if outs is None:
+        outs = [torch.empty((d, R), dtype=torch.float32, device=factors[0].device) for d in info["dims"]]
+    for o, d in zip(outs, info["dims"]):
+        if tuple(o.shape) != (d, R) or o.dtype != torch.float32 or not o.is_contiguous():
+            raise ValueError(f"outputs must be contiguous float32 tensors of shape (dims[m], {R})")
+    op = (VP * len(outs))(*[VP(o.data_ptr()) for o in outs])
+    ms = (C.c_float * len(outs))() if timed else None
+    _check(lib().mmcsf_mttkrp_allmode(fmt._h, int(R), _dtype_of(factors), _factor_ptrs(factors), op,
+                                      _stream_ptr(stream), ms))
+    return outs, ([float(v) for v in ms] if timed else None)

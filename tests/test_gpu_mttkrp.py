@@ -1,0 +1,195 @@
+"""GPU MTTKRP (COO / B-CSF / HB-CSF / MM-CSF, 3-D and 4-D, FP32 and BF16
+factor storage) against the CPU oracle.
+
+Tolerances (written per test):
+* vs the float64 oracle: ||M - M64||_F / ||M64||_F <= 1e-6 with float32
+  factors (float32 accumulation of short sums);
+* vs the reference's sequential float32 COO CPU MTTKRP (-t 1) on the
+  uniform 2000^3 / 1M-nonzero / R=32 workload: <= 1e-5 relative;
+* BF16 storage: <= 1e-3 relative against float64 of the float32 factors
+  (quantisation included), <= 1e-6 against float64 of the bf16 factors.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+from test_gpu_builder import random_tensor  # noqa: E402
+
+
+def rel(a, b):
+    b = np.asarray(b, dtype=np.float64)
+    n = np.linalg.norm(b)
+    return float(np.linalg.norm(np.asarray(a, np.float64) - b) / (n if n > 0 else 1.0))
+
+
+def factors(rng, dims, R, torch, dev):
+    U = [rng.random((d, R)).astype(np.float32) for d in dims]
+    return U, [torch.from_numpy(u).to(dev) for u in U]
+
+
+def bf16_round(U, torch):
+    return [torch.from_numpy(u).to(torch.bfloat16).to(torch.float32).numpy() for u in U]
+
+
+def test_toy_closed_form_every_type(cuda, golden_dir):
+    import torch
+    import mmcsf
+    t = mmcsf.Tensor.load_tns(f"{golden_dir}/toy.tns")
+    expect = {0: [48.75, 67.5, 52.5], 1: [13.75, 16.25, 26.25], 2: [12.75, 13.5, 7.5]}
+    for R in (1, 4, 32, 33):
+        Ud = [torch.full((3, R), m + 0.5, device=cuda) for m in range(3)]
+        fm = mmcsf.Format(t, 12, fbr_threshold=128)
+        outs, _ = mmcsf.mttkrp_allmode(fm, Ud)
+        for mode in range(3):
+            np.testing.assert_array_equal(outs[mode].cpu().numpy(), np.repeat(np.array(expect[mode], np.float32)[:, None], R, 1))
+            for ft in (3, 8, 10):
+                f = mmcsf.Format(t, ft, mode=mode, fbr_threshold=128)
+                o = mmcsf.mttkrp(f, mode, Ud).cpu().numpy()
+                np.testing.assert_array_equal(o, np.repeat(np.array(expect[mode], np.float32)[:, None], R, 1))
+
+
+CASES = [
+    ([40, 30, 20], 4000, None),
+    ([500, 200, 50], 30000, [1.2, 1.2, None]),
+    ([300, 40, 6], 8000, [1.3, 1.2, None]),
+    ([20, 25, 30, 15], 8000, None),
+    ([100, 50, 20, 10], 40000, [1.1, None, 1.1, None]),
+]
+
+
+@pytest.mark.parametrize("dims,nnz,skew", CASES)
+@pytest.mark.parametrize("R", [1, 3, 8, 16, 32, 64, 100, 256])
+def test_all_types_vs_float64(cuda, dims, nnz, skew, R):
+    import torch
+    import mmcsf
+    rng = np.random.default_rng(nnz + R)
+    inds, vals = random_tensor(rng, dims, nnz, skew)
+    U, Ud = factors(rng, dims, R, torch, cuda)
+    t = mmcsf.Tensor.from_numpy(dims, inds, vals)
+    N = len(dims)
+    ref = [O.mttkrp_coo_f64(inds, vals, m, U) for m in range(N)]
+    for f in (2, 128):
+        fm = mmcsf.Format(t, 12, fbr_threshold=f)
+        outs, _ = mmcsf.mttkrp_allmode(fm, Ud)
+        for m in range(N):
+            assert rel(outs[m].cpu().numpy(), ref[m]) < 1e-6, f"MM-CSF mode {m} f={f}"
+            one = mmcsf.mttkrp(fm, m, Ud).cpu().numpy()
+            assert rel(one, ref[m]) < 1e-6
+    for m in range(N):
+        for ft in (3, 8, 10):
+            f = mmcsf.Format(t, ft, mode=m, fbr_threshold=3)
+            o = mmcsf.mttkrp(f, m, Ud).cpu().numpy()
+            assert rel(o, ref[m]) < 1e-6, f"type {ft} mode {m}"
+
+
+@pytest.mark.parametrize("dims,nnz,skew", CASES[:2] + CASES[3:4])
+@pytest.mark.parametrize("R", [16, 32, 64])
+def test_bf16_factor_storage(cuda, dims, nnz, skew, R):
+    import torch
+    import mmcsf
+    rng = np.random.default_rng(7 * nnz + R)
+    inds, vals = random_tensor(rng, dims, nnz, skew)
+    U, Ud = factors(rng, dims, R, torch, cuda)
+    Ub = [u.to(torch.bfloat16) for u in Ud]
+    Ur = bf16_round(U, torch)
+    t = mmcsf.Tensor.from_numpy(dims, inds, vals)
+    fm = mmcsf.Format(t, 12, fbr_threshold=128)
+    outs, _ = mmcsf.mttkrp_allmode(fm, Ub)
+    for m in range(len(dims)):
+        got = outs[m].cpu().numpy()
+        assert rel(got, O.mttkrp_coo_f64(inds, vals, m, U)) < 1e-3
+        assert rel(got, O.mttkrp_coo_f64(inds, vals, m, Ur)) < 1e-6
+    for ft in (3, 8, 10):
+        f = mmcsf.Format(t, ft, mode=1, fbr_threshold=128)
+        got = mmcsf.mttkrp(f, 1, Ub).cpu().numpy()
+        assert rel(got, O.mttkrp_coo_f64(inds, vals, 1, Ur)) < 1e-6
+
+
+def test_uniform_2000_cubed_vs_t1_reference(cuda):
+    """The plain mode-0 workload: uniform 2000^3, 1M distinct nonzeros, R=32,
+    -f 128; GPU COO and B-CSF (and MM-CSF) against the -t 1 sequential float32
+    restatement within 1e-5 relative, and against float64."""
+    import torch
+    import mmcsf
+    dims = [2000, 2000, 2000]
+    t = mmcsf.Tensor.generate(dims, 1_000_000, [0, 0, 0], [0, 0, 0], seed=0x4D4D43534600)
+    _, inds, vals = t.to_numpy()
+    g = torch.Generator(device="cpu").manual_seed(1)
+    U = [torch.rand((d, 32), generator=g).numpy() for d in dims]
+    Ud = [torch.from_numpy(u).to(cuda) for u in U]
+    order = O.order_tensormode(dims, 0)
+    perm = O.sort_perm(inds, order)  # sort_COOtensor before -t 1 (mttkrp.cu:79)
+    sinds = [a[perm] for a in inds]
+    t1 = np.zeros((dims[0], 32), np.float32)
+    O.mttkrp_coo_f32(sinds, vals[perm], 0, U, t1)
+    r64 = O.mttkrp_coo_f64(inds, vals, 0, U)
+    assert rel(t1, r64) < 1e-6  # -t 1 is itself accurate on ~500-term rows
+    for ft in (3, 8, 10, 12):
+        f = mmcsf.Format(t, ft, mode=0, fbr_threshold=128)
+        got = mmcsf.mttkrp(f, 0, Ud).cpu().numpy()
+        assert rel(got, t1) < 1e-5, f"type {ft}"
+        assert rel(got, r64) < 1e-6, f"type {ft}"
+
+
+def test_empty_tensor_gives_zero_output(cuda):
+    import torch
+    import mmcsf
+    dims = [4, 5, 6]
+    t = mmcsf.Tensor.from_numpy(dims, [np.zeros(0, np.int64)] * 3, np.zeros(0, np.float32))
+    Ud = [torch.rand((d, 8), device=cuda) for d in dims]
+    for ft in (3, 8, 12):
+        f = mmcsf.Format(t, ft, mode=0)
+        out = torch.full((4, 8), 7.0, device=cuda)
+        mmcsf.mttkrp(f, 0, Ud, out=out)
+        assert float(out.abs().max()) == 0.0
+
+
+def test_repeat_runs_agree(cuda):
+    import torch
+    import mmcsf
+    rng = np.random.default_rng(21)
+    dims = [500, 200, 50]
+    inds, vals = random_tensor(rng, dims, 30000, [1.2, 1.2, None])
+    U, Ud = factors(rng, dims, 32, torch, cuda)
+    t = mmcsf.Tensor.from_numpy(dims, inds, vals)
+    fm = mmcsf.Format(t, 12, fbr_threshold=128)
+    a, _ = mmcsf.mttkrp_allmode(fm, Ud)
+    b, _ = mmcsf.mttkrp_allmode(fm, Ud)
+    for m in range(3):
+        assert rel(a[m].cpu().numpy(), b[m].cpu().numpy()) < 1e-6
+
+
+def test_error_paths(cuda):
+    import torch
+    import mmcsf
+    rng = np.random.default_rng(1)
+    dims = [10, 9, 8]
+    inds, vals = random_tensor(rng, dims, 300)
+    t = mmcsf.Tensor.from_numpy(dims, inds, vals)
+    Ud = [torch.rand((d, 8), device=cuda) for d in dims]
+    fm = mmcsf.Format(t, 12)
+    with pytest.raises(mmcsf.MMCSFError):
+        mmcsf.mttkrp(fm, 3, Ud)                                  # mode out of range
+    with pytest.raises(mmcsf.MMCSFError):
+        mmcsf.mttkrp(fm, 0, [torch.rand((d, 257), device=cuda) for d in dims])   # R > 256
+    with pytest.raises(mmcsf.MMCSFError):
+        mmcsf.mttkrp(fm, 0, [torch.rand((d, 6), device=cuda).to(torch.bfloat16) for d in dims])
+    fb = mmcsf.Format(t, 8, mode=1)
+    with pytest.raises(mmcsf.MMCSFError):
+        mmcsf.mttkrp(fb, 0, Ud)                                  # B-CSF serves its root mode only
+    with pytest.raises(mmcsf.MMCSFError):
+        mmcsf.Tensor.from_numpy([10, 9], inds[:2], vals)          # 2 modes
+    with pytest.raises(mmcsf.MMCSFError):
+        mmcsf.Tensor.from_numpy(dims, [inds[0] + 10, inds[1], inds[2]], vals)  # out of range
+    with pytest.raises(mmcsf.MMCSFError):
+        mmcsf.Tensor.load_tns("/nonexistent/file.tns")
+    with pytest.raises(mmcsf.MMCSFError):
+        mmcsf.Format(t, 7)                                       # unsupported type
+    # misaligned factor pointer
+    big = torch.rand(10 * 8 + 1, device=cuda)
+    mis = big[1:].view(10, 8)
+    with pytest.raises(mmcsf.MMCSFError):
+        mmcsf.mttkrp(fm, 1, [mis, None, Ud[2]])
