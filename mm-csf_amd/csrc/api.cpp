@@ -384,6 +384,8 @@ int mmcsf_format_build(const mmcsf_tensor *th, int type, const mmcsf_options *op
             sort_coo(t, order, keys, vals, s);
             uint64_t fs = t.ndims == 4 ? 128 : kNoSplit;  // util.h:1325-1326
             build_csf(t.ndims, t.dims, t.nnz, order, keys, std::move(vals), opt->fbrThreashold, fs, f.csf, s);
+            // privatisation tables of the scatter levels (every level but the root)
+            for (int l = 1; l < t.ndims; ++l) build_hot_table(f.csf, l, t.dims[order[l]], f.csf.hot[l], s);
         } else {
             order_tensormode(t.ndims, t.dims, (int)opt->mode, nat, order);
             if (type == MMCSF_TYPE_COO || type == MMCSF_TYPE_COO_CPU) {

@@ -796,4 +796,67 @@ void build_hyb(int N, const uint64_t *dims, Csf &&src, Hyb &h, hipStream_t s) {
     (void)dead;
 }
 
+
+// ---------------------------------------------------------------------------
+// Hot-row tables of a scatter level: the rows of level `level` used most
+// often (by nonzeros for the leaf level, by tree entries for fiber levels)
+// are ranked; the level's index array is re-encoded so that a row of rank
+// r < kMaxHot reads kHotBit | r and every other row reads its coordinate.
+// The scatter kernels accumulate ranked rows in LDS.
+// ---------------------------------------------------------------------------
+namespace {
+
+__global__ void k_hist(const uint32_t *__restrict__ idx, uint64_t n, uint32_t *__restrict__ cnt) {
+    uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (i < n) atomicAdd(&cnt[idx[i]], 1u);
+}
+
+__global__ void k_rank_table(const uint32_t *__restrict__ rowsByCount, uint32_t nhot,
+                             uint32_t *__restrict__ rank) {
+    uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < nhot) rank[rowsByCount[r]] = kHotBit | r;
+}
+
+__global__ void k_encode(const uint32_t *__restrict__ idx, const uint32_t *__restrict__ rank, uint64_t n,
+                         uint32_t *__restrict__ enc) {
+    uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t v = idx[i];
+    uint32_t r = rank[v];
+    enc[i] = r != 0xffffffffu ? r : v;
+}
+
+}  // namespace
+
+void build_hot_table(const Csf &c, int level, uint64_t rows, HotTable &h, hipStream_t s) {
+    const uint32_t *idx = level == c.ndims - 1 ? c.leaf.p : c.idx[level].p;
+    const uint64_t n = level == c.ndims - 1 ? c.nnz : c.cnt[level];
+    Temp tmp;
+    h.count = 0;
+    h.tgt.alloc(n);
+    if (n == 0 || rows == 0) return;
+    DevArray<uint32_t> cnt(rows), cnt2(rows), ids(rows), ids2(rows), rank(rows);
+    MMCSF_HIP(hipMemsetAsync(cnt.p, 0, rows * 4, s));
+    hipLaunchKernelGGL(k_hist, dim3(nblk(n)), dim3(kBS), 0, s, idx, n, cnt.p);
+    hipLaunchKernelGGL(k_iota, dim3(nblk(rows)), dim3(kBS), 0, s, ids.p, rows);
+    size_t bytes = 0;
+    MMCSF_HIP(rocprim::radix_sort_pairs_desc(nullptr, bytes, cnt.p, cnt2.p, ids.p, ids2.p, (size_t)rows, 0u,
+                                             32u, s));
+    MMCSF_HIP(rocprim::radix_sort_pairs_desc(tmp.get(bytes), bytes, cnt.p, cnt2.p, ids.p, ids2.p,
+                                             (size_t)rows, 0u, 32u, s));
+    // rows with at least one use, capped at kMaxHot
+    std::vector<uint32_t> top(std::min<uint64_t>(rows, kMaxHot));
+    MMCSF_HIP(hipMemcpyAsync(top.data(), cnt2.p, top.size() * 4, hipMemcpyDeviceToHost, s));
+    MMCSF_HIP(hipStreamSynchronize(s));
+    uint32_t nhot = 0;
+    while (nhot < top.size() && top[nhot] > 0) ++nhot;
+    h.count = nhot;
+    h.rows.alloc(std::max<uint32_t>(nhot, 1));
+    if (nhot) MMCSF_HIP(hipMemcpyAsync(h.rows.p, ids2.p, nhot * 4, hipMemcpyDeviceToDevice, s));
+    MMCSF_HIP(hipMemsetAsync(rank.p, 0xff, rows * 4, s));
+    if (nhot) hipLaunchKernelGGL(k_rank_table, dim3((nhot + 255) / 256), dim3(256), 0, s, ids2.p, nhot, rank.p);
+    hipLaunchKernelGGL(k_encode, dim3(nblk(n)), dim3(kBS), 0, s, idx, rank.p, n, h.tgt.p);
+    MMCSF_HIP(hipStreamSynchronize(s));
+}
+
 }  // namespace mmcsf

@@ -77,6 +77,7 @@ struct DevArray {
 constexpr int kMaxDims = 4;
 constexpr int kNumBins = 10;             // Options::nBin, util.h:205
 constexpr uint32_t kHotBit = 0x80000000u;  // target-array tag of a privatised row
+constexpr uint32_t kMaxHot = 4096;         // ranked rows per scatter level
 
 // COO tensor: class Tensor (util.h:80-108) without the derived members.
 struct Tensor {
@@ -89,7 +90,7 @@ struct Tensor {
 
 // Top-k rows of a scatter output, privatised in LDS by the scatter kernels.
 struct HotTable {
-    uint32_t count = 0;                  // rows in the table (<= kMaxHot)
+    uint32_t count = 0;                  // ranked rows (<= kMaxHot)
     DevArray<uint32_t> rows;             // rank -> row
     DevArray<uint32_t> tgt;              // per contribution: row, or kHotBit|rank
 };
@@ -114,8 +115,7 @@ struct Csf {
     uint64_t binCount[kNumBins] = {0};
     uint64_t dropped = 0;
     // privatisation tables for the scatter roles, indexed by output level
-    HotTable hot[kMaxDims];
-    bool hotBuilt[kMaxDims] = {false, false, false, false};
+    HotTable hot[kMaxDims];               // built for levels 1..N-1 of MM-CSF
 };
 
 // HB-CSF parts (HYBTensor, util.h:141-173)

@@ -27,165 +27,203 @@ using dev::upper_index;
 constexpr int kBlock = 256;
 
 // --------------------------------------------------------------------------
-// CSF walker: N = 3 or 4 levels of coordinates, OUT = level of the output mode
+// CSF walker: N = 3 or 4 levels of coordinates, OUT = level of the output mode.
+// Persistent: every group of G lanes owns one contiguous, equal share of the
+// nonzeros.  Scatter roles (OUT > 0) accumulate the top-ranked output rows in
+// LDS and flush them once per block; other rows take global atomics.
 // --------------------------------------------------------------------------
+constexpr int kMaxBlock = 1024;
+
 template <int N, int OUT, int G, int NV, int VEC, int B, typename T>
-__global__ __launch_bounds__(kBlock) void k_csf(CsfArgs a) {
+__global__ __launch_bounds__(kMaxBlock) void k_csf(CsfArgs a) {
+    extern __shared__ float lds[];
     using RowT = Row<G, NV, VEC>;
     constexpr int LF = N - 2;  // leaf-fiber level
-    const int lane = threadIdx.x % G;
-    const uint64_t grp = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / G;
-    const uint64_t x0l = grp * (uint64_t)a.perGroup;
-    if (x0l >= a.nnz) return;
-    const uint32_t x0 = (uint32_t)x0l;
-    const uint32_t x1 = (uint32_t)min((uint64_t)a.nnz, x0l + a.perGroup);
+    constexpr bool SCATTER = OUT > 0;
     const int R = a.R;
-    const T *U0 = static_cast<const T *>(a.U[0]);
-    const T *U1 = static_cast<const T *>(a.U[1]);
-    const T *U2 = static_cast<const T *>(a.U[2]);
-    const T *UL = static_cast<const T *>(a.U[N - 1]);
-    const uint32_t *__restrict__ ptrL = a.ptr[LF];
-    const uint32_t *__restrict__ leaf = a.leaf;
-    const float *__restrict__ val = a.val;
+    const uint32_t nH = SCATTER ? a.nHot : 0u;
+    const uint32_t stride = a.ldsStride;
+    if constexpr (SCATTER) {
+        for (uint32_t i = threadIdx.x; i < nH * stride; i += blockDim.x) lds[i] = 0.f;
+        __syncthreads();
+    }
+    const int lane = threadIdx.x % G;
+    const uint64_t ngroups = (uint64_t)gridDim.x * blockDim.x / G;
+    const uint64_t grp = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / G;
+    const uint32_t x0 = (uint32_t)(grp * a.nnz / ngroups);
+    const uint32_t x1 = (uint32_t)((grp + 1) * a.nnz / ngroups);
     float *__restrict__ out = a.out;
 
-    uint32_t f = upper_index(ptrL, a.cnt[LF], x0);
-    uint32_t fend = ptrL[f + 1];
-    uint32_t j = f, jend = 0;
-    if constexpr (N == 4) {
-        j = upper_index(a.ptr[1], a.cnt[1], f);
-        jend = a.ptr[1][j + 1];
-    }
-    uint32_t slc = a.slc[j];
-
-    RowT aF, aJ, aS, PS, PJ, P, t;
-    aF.zero();
-    aJ.zero();
-    aS.zero();
-    PS.zero();
-    PJ.zero();
-    P.zero();
-
-    auto begin_slice = [&]() {
-        if constexpr (OUT >= 1) PS.load(U0, slc, R, lane);
-    };
-    auto begin_j = [&]() {  // 4-D: product of slice and fbrS rows
-        if constexpr (N == 4 && OUT >= 2) {
-            t.load(U1, a.idx[1][j], R, lane);
-            PJ.had(PS, t);
-        }
-    };
-    auto begin_fiber = [&]() {
-        if constexpr (OUT == N - 1) {
-            if constexpr (N == 3) {
-                t.load(U1, a.idx[1][f], R, lane);
-                P.had(PS, t);
-            } else {
-                t.load(U2, a.idx[2][f], R, lane);
-                P.had(PJ, t);
+    // out[target] += v, through LDS for ranked rows
+    auto scatter = [&](uint32_t tgt, const RowT &v) {
+        if (tgt & kHotBit) {
+            uint32_t r = tgt & ~kHotBit;
+            if (r < nH) {
+                v.lds_add(lds + (size_t)r * stride, R, lane);
+                return;
             }
+            tgt = a.hotRows[r];
         }
+        v.atomic_add(out, tgt, R, lane);
     };
-    auto end_fiber = [&]() {
-        if constexpr (N == 3 && OUT == 0) {
-            t.load(U1, a.idx[1][f], R, lane);
-            aS.fma_had(aF, t);
-        } else if constexpr (N == 3 && OUT == 1) {
-            aF.mul(PS);
-            aF.atomic_add(out, a.idx[1][f], R, lane);
-        } else if constexpr (N == 4 && (OUT == 0 || OUT == 1)) {
-            t.load(U2, a.idx[2][f], R, lane);
-            aJ.fma_had(aF, t);
-        } else if constexpr (N == 4 && OUT == 2) {
-            aF.mul(PJ);
-            aF.atomic_add(out, a.idx[2][f], R, lane);
+
+    if (x0 < x1) {
+        const T *U0 = static_cast<const T *>(a.U[0]);
+        const T *U1 = static_cast<const T *>(a.U[1]);
+        const T *U2 = static_cast<const T *>(a.U[2]);
+        const T *UL = static_cast<const T *>(a.U[N - 1]);
+        const uint32_t *__restrict__ ptrL = a.ptr[LF];
+        const uint32_t *__restrict__ leaf = (OUT == N - 1) ? a.tgt : a.leaf;
+        const uint32_t *__restrict__ tgtF = a.tgt;
+        const float *__restrict__ val = a.val;
+
+        uint32_t f = upper_index(ptrL, a.cnt[LF], x0);
+        uint32_t fend = ptrL[f + 1];
+        uint32_t j = f, jend = 0;
+        if constexpr (N == 4) {
+            j = upper_index(a.ptr[1], a.cnt[1], f);
+            jend = a.ptr[1][j + 1];
         }
+        uint32_t slc = a.slc[j];
+
+        RowT aF, aJ, aS, PS, PJ, P, t;
         aF.zero();
-    };
-    auto end_j = [&]() {
-        if constexpr (N == 4 && OUT == 0) {
-            t.load(U1, a.idx[1][j], R, lane);
-            aS.fma_had(aJ, t);
-            aJ.zero();
-        } else if constexpr (N == 4 && OUT == 1) {
-            aJ.mul(PS);
-            aJ.atomic_add(out, a.idx[1][j], R, lane);
-            aJ.zero();
-        }
-    };
-    auto end_slice = [&]() {
-        if constexpr (OUT == 0) {
-            aS.atomic_add(out, slc, R, lane);
-            aS.zero();
-        }
-    };
+        aJ.zero();
+        aS.zero();
+        PS.zero();
+        PJ.zero();
+        P.zero();
 
-    begin_slice();
-    begin_j();
-    begin_fiber();
-    for (uint32_t xb = x0; xb < x1; xb += B) {
-        uint32_t li[B];
-        float vv[B];
-        RowT rows[B];
-#pragma unroll
-        for (int k = 0; k < B; ++k) {
-            uint32_t x = xb + k;
-            li[k] = 0;
-            vv[k] = 0.f;
-            if (x < x1) {
-                li[k] = leaf[x];
-                vv[k] = val[x];
+        auto begin_slice = [&]() {
+            if constexpr (OUT >= 1) PS.load(U0, slc, R, lane);
+        };
+        auto begin_j = [&]() {  // 4-D: product of slice and fbrS rows
+            if constexpr (N == 4 && OUT >= 2) {
+                t.load(U1, a.idx[1][j], R, lane);
+                PJ.had(PS, t);
             }
-        }
-        if constexpr (OUT != N - 1) {
+        };
+        auto begin_fiber = [&]() {
+            if constexpr (OUT == N - 1) {
+                if constexpr (N == 3) {
+                    t.load(U1, a.idx[1][f], R, lane);
+                    P.had(PS, t);
+                } else {
+                    t.load(U2, a.idx[2][f], R, lane);
+                    P.had(PJ, t);
+                }
+            }
+        };
+        auto end_fiber = [&]() {
+            if constexpr (N == 3 && OUT == 0) {
+                t.load(U1, a.idx[1][f], R, lane);
+                aS.fma_had(aF, t);
+            } else if constexpr (N == 3 && OUT == 1) {
+                aF.mul(PS);
+                scatter(tgtF[f], aF);
+            } else if constexpr (N == 4 && (OUT == 0 || OUT == 1)) {
+                t.load(U2, a.idx[2][f], R, lane);
+                aJ.fma_had(aF, t);
+            } else if constexpr (N == 4 && OUT == 2) {
+                aF.mul(PJ);
+                scatter(tgtF[f], aF);
+            }
+            aF.zero();
+        };
+        auto end_j = [&]() {
+            if constexpr (N == 4 && OUT == 0) {
+                t.load(U1, a.idx[1][j], R, lane);
+                aS.fma_had(aJ, t);
+                aJ.zero();
+            } else if constexpr (N == 4 && OUT == 1) {
+                aJ.mul(PS);
+                scatter(tgtF[j], aJ);
+                aJ.zero();
+            }
+        };
+        auto end_slice = [&]() {
+            if constexpr (OUT == 0) {
+                aS.atomic_add(out, slc, R, lane);
+                aS.zero();
+            }
+        };
+
+        begin_slice();
+        begin_j();
+        begin_fiber();
+        for (uint32_t xb = x0; xb < x1; xb += B) {
+            uint32_t li[B];
+            float vv[B];
+            RowT rows[B];
 #pragma unroll
-            for (int k = 0; k < B; ++k)
-                if (xb + k < x1) rows[k].load(UL, li[k], R, lane);
-        }
+            for (int k = 0; k < B; ++k) {
+                uint32_t x = xb + k;
+                li[k] = 0;
+                vv[k] = 0.f;
+                if (x < x1) {
+                    li[k] = leaf[x];
+                    vv[k] = val[x];
+                }
+            }
+            if constexpr (OUT != N - 1) {
 #pragma unroll
-        for (int k = 0; k < B; ++k) {
-            uint32_t x = xb + k;
-            if (x >= x1) break;
-            if (x == fend) {
-                end_fiber();
-                ++f;
-                fend = ptrL[f + 1];
-                if constexpr (N == 4) {
-                    if (f == jend) {
-                        end_j();
-                        ++j;
-                        jend = a.ptr[1][j + 1];
-                        uint32_t ns = a.slc[j];
+                for (int k = 0; k < B; ++k)
+                    if (xb + k < x1) rows[k].load(UL, li[k], R, lane);
+            }
+#pragma unroll
+            for (int k = 0; k < B; ++k) {
+                uint32_t x = xb + k;
+                if (x >= x1) break;
+                if (x == fend) {
+                    end_fiber();
+                    ++f;
+                    fend = ptrL[f + 1];
+                    if constexpr (N == 4) {
+                        if (f == jend) {
+                            end_j();
+                            ++j;
+                            jend = a.ptr[1][j + 1];
+                            uint32_t ns = a.slc[j];
+                            if (ns != slc) {
+                                end_slice();
+                                slc = ns;
+                                begin_slice();
+                            }
+                            begin_j();
+                        }
+                    } else {
+                        uint32_t ns = a.slc[f];
                         if (ns != slc) {
                             end_slice();
                             slc = ns;
                             begin_slice();
                         }
-                        begin_j();
                     }
-                } else {
-                    uint32_t ns = a.slc[f];
-                    if (ns != slc) {
-                        end_slice();
-                        slc = ns;
-                        begin_slice();
-                    }
+                    begin_fiber();
                 }
-                begin_fiber();
-            }
-            if constexpr (OUT == N - 1) {
-                RowT c;
-                c.zero();
-                c.fma(vv[k], P);
-                c.atomic_add(out, li[k], R, lane);
-            } else {
-                aF.fma(vv[k], rows[k]);
+                if constexpr (OUT == N - 1) {
+                    RowT c;
+                    c.zero();
+                    c.fma(vv[k], P);
+                    scatter(li[k], c);
+                } else {
+                    aF.fma(vv[k], rows[k]);
+                }
             }
         }
+        end_fiber();
+        end_j();
+        end_slice();
     }
-    end_fiber();
-    end_j();
-    end_slice();
+
+    if constexpr (SCATTER) {
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < nH * (uint32_t)R; i += blockDim.x) {
+            uint32_t r = i / (uint32_t)R, c = i - r * (uint32_t)R;
+            float v = lds[(size_t)r * stride + c];
+            if (v != 0.f) unsafeAtomicAdd(out + (size_t)a.hotRows[r] * R + c, v);
+        }
+    }
 }
 
 // --------------------------------------------------------------------------
@@ -365,15 +403,6 @@ void run_geo(const Geo &g, Args &a, uint64_t groups, hipStream_t s) {
 }
 
 // kernel selectors with a uniform template signature
-template <int N, int OUT>
-struct CsfSel {
-    template <int G, int NV, int VEC, int B, typename T>
-    struct K {
-        static void launch(dim3 g, dim3 b, hipStream_t s, const CsfArgs &a) {
-            k_csf<N, OUT, G, NV, VEC, B, T><<<g, b, 0, s>>>(a);
-        }
-    };
-};
 template <int N>
 struct CooSel {
     template <int G, int NV, int VEC, int B, typename T>
@@ -391,12 +420,41 @@ struct CslK {
 };
 
 template <int N, int OUT>
+struct CsfLaunch {
+    template <int G, int NV, int VEC, int B, typename T>
+    struct K {
+        static void launch(dim3, dim3, hipStream_t s, const CsfArgs &a) {
+            auto fn = k_csf<N, OUT, G, NV, VEC, B, T>;
+            const size_t lds = (size_t)a.nHot * a.ldsStride * sizeof(float);
+            const int block = OUT > 0 ? kMaxBlock : 256;
+            if (lds > 65536) MMCSF_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(fn),
+                                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+            int perCU = 0;
+            MMCSF_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, fn, block, lds));
+            perCU = std::max(perCU, 1);
+            dim3 grid((unsigned)(device_cu_count() * perCU));
+            fn<<<grid, dim3(block), lds, s>>>(a);
+        }
+    };
+};
+
+template <int N, int OUT>
 void launch_csf_t(const Geo &g, CsfArgs &a, int dtype, hipStream_t s) {
-    uint64_t groups = (a.nnz + a.perGroup - 1) / a.perGroup;
     if (dtype == MMCSF_F32)
-        run_geo<CsfSel<N, OUT>::template K, CsfArgs, float>(g, a, groups, s);
+        run_geo<CsfLaunch<N, OUT>::template K, CsfArgs, float>(g, a, 1, s);
     else
-        run_geo<CsfSel<N, OUT>::template K, CsfArgs, uint16_t>(g, a, groups, s);
+        run_geo<CsfLaunch<N, OUT>::template K, CsfArgs, uint16_t>(g, a, 1, s);
+}
+
+int max_lds_per_block() {
+    static int bytes = 0;
+    if (!bytes) {
+        int dev = 0;
+        MMCSF_HIP(hipGetDevice(&dev));
+        MMCSF_HIP(hipDeviceGetAttribute(&bytes, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+        if (bytes <= 0) bytes = 65536;
+    }
+    return bytes;
 }
 
 void check_align(const void *p, int dtype, const Geo &g) {
@@ -461,6 +519,16 @@ void mttkrp_csf(const Csf &c, const uint64_t *dims, int outLevel, const Factors 
     a.nnz = (uint32_t)c.nnz;
     a.R = F.R;
     a.perGroup = per_group_for(c.nnz, g.G);
+    if (outLevel > 0) {
+        const HotTable &h = c.hot[outLevel];
+        MMCSF_REQUIRE(h.tgt.p != nullptr || c.nnz == 0, MMCSF_ERR_ARG,
+                      "this format serves its root mode only");
+        a.tgt = h.tgt.p;
+        a.hotRows = h.rows.p;
+        a.ldsStride = (uint32_t)F.R + 1;
+        const uint64_t cap = (uint64_t)(max_lds_per_block() - 1024) / (a.ldsStride * sizeof(float));
+        a.nHot = (uint32_t)std::min<uint64_t>(h.count, cap);
+    }
     if (N == 3) {
         switch (outLevel) {
             case 0: launch_csf_t<3, 0>(g, a, F.dtype, s); break;

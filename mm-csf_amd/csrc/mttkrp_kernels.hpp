@@ -122,6 +122,17 @@ struct Row {
             }
         }
     }
+    // lds[c] += this[c] for an LDS-resident row (ds_add_f32)
+    __device__ __forceinline__ void lds_add(float *base, int R, int lane) const {
+#pragma unroll
+        for (int a = 0; a < NV; ++a) {
+            int c = (a * G + lane) * VEC;
+            if (c < R) {
+#pragma unroll
+                for (int b = 0; b < VEC; ++b) atomicAdd(base + c + b, v[a][b]);
+            }
+        }
+    }
     // out[row] = this (rows owned by one group)
     __device__ __forceinline__ void store(float *__restrict__ out, uint32_t row, int R, int lane) const {
         float *base = out + (size_t)row * (size_t)R;
@@ -163,8 +174,13 @@ struct CsfArgs {
     float *out;
     uint32_t nnz;
     uint32_t cnt[4];
-    uint32_t perGroup;        // nonzeros per row group
+    uint32_t perGroup;        // nonzeros per row group (non-persistent launches)
     int R;
+    // scatter roles: output-level index with ranked rows tagged kHotBit|rank
+    const uint32_t *tgt;
+    const uint32_t *hotRows;  // rank -> row
+    uint32_t nHot;            // ranks accumulated in LDS by this launch
+    uint32_t ldsStride;       // floats per LDS row
 };
 
 }  // namespace mmcsf
