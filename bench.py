@@ -56,27 +56,29 @@ def algorithmic_bytes(info, distinct, R, sf):
 
 def fp64_reference(inds, vals, U, mode, dims, chunk=4_000_000):
     """float64 MTTKRP on the GPU: contributions sorted by output row are
-    reduced per row segment (prefix sums), so hot rows need no atomics."""
+    reduced per row segment with prefix sums along the nonzero axis (no
+    atomics on hot rows)."""
     import torch
     R = U[0].shape[1]
     ref = torch.zeros((dims[mode], R), dtype=torch.float64, device=vals.device)
-    U64 = [u.double() for u in U]
+    UT = [u.double().t().contiguous() for u in U]          # R x dims[m]
     order = torch.argsort(inds[mode], stable=True)
     n = vals.numel()
     for s in range(0, n, chunk):
         sel = order[s:min(s + chunk, n)]
         rows = inds[mode][sel].long()
-        c = vals[sel].double()[:, None].expand(-1, R).clone()
+        c = vals[sel].double()[None, :].expand(R, -1).clone()   # R x chunk
         for m in range(len(dims)):
             if m != mode:
-                c *= U64[m][inds[m][sel].long()]
-        cs = torch.cumsum(c, dim=0)
+                c *= UT[m][:, inds[m][sel].long()]
+        cs = torch.cumsum(c, dim=1)
         last = torch.nonzero(rows[1:] != rows[:-1]).flatten()
         last = torch.cat([last, torch.tensor([rows.numel() - 1], device=rows.device)])
-        seg = cs[last]
-        seg[1:] -= cs[last[:-1]]
-        ref.index_add_(0, rows[last], seg)
+        seg = cs[:, last]
+        seg[:, 1:] -= cs[:, last[:-1]]
+        ref.index_add_(0, rows[last], seg.t())
     return ref
+
 
 def rel_err(out, ref):
     import torch
@@ -129,7 +131,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--nnz", type=int, default=100_000_000)
-    ap.add_argument("--cpu-sample", type=int, default=20_000_000)
+    ap.add_argument("--cpu-sample", type=int, default=50_000_000)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     args = ap.parse_args()
     if args.gpus != 1:

@@ -61,10 +61,11 @@ typedef struct mmcsf_options {
     int verbose;              /* -v 1 */
     int correctness;          /* -c 1 */
     uint64_t nBin;            /* 10 (no flag) */
-    uint64_t cpdIters;        /* 10 (no flag in the reference) */
+    uint64_t cpdIters;        /* -n CP-ALS iterations (10; no flag in the reference) */
     int natOrdering;          /* 0 (no flag) */
     uint64_t fbrThreashold;   /* -f fiber split threshold (99999999) */
     int factorDtype;          /* -d 0|1: FP32 / BF16 factor storage (new flag) */
+    uint64_t seed;            /* -x seed of generated inputs (new flag, 1) */
     char inFileName[4096];    /* -i */
     char outFileName[4096];   /* -o */
     char m0[16], m1[16], m2[16];  /* -p/-q/-r, unused like the reference */
@@ -224,6 +225,39 @@ int mmcsf_mttkrp_allmode(const mmcsf_format *f, int R, int dtype,
  * type, selected explicitly; no GPU entry point ever falls back to it. */
 int mmcsf_mttkrp_coo_host(const mmcsf_format *f, int mode, int R,
                           const float *const *factors_host, float *out_host);
+
+/* ---- CP-ALS ------------------------------------------------------------------- */
+
+typedef struct mmcsf_cpd_stats {
+    int iterations;            /* ALS iterations run */
+    double fit;                /* fit after the last iteration */
+    double fit_history[64];    /* fit of iterations 0..63 */
+    double ms_total;           /* ALS loop, host clock */
+    double ms_mttkrp;          /* MTTKRP kernels, HIP events */
+    int lu_fallbacks;          /* solves that fell back from Cholesky to LU */
+} mmcsf_cpd_stats;
+
+/*
+ * CP-ALS on an MM-CSF format, entirely on the device (cpd(), cpd_cpu.h:62,
+ * cpd_cpu.cu:484-691, with init_GPU and MTTKRP_MIHCSR_GPU_oneMode_forCPD,
+ * mttkrp_gpu.h:77-81).  factors[m]: device float32 dims[m] x R, read as the
+ * initial guess and overwritten with the final column-normalised factors;
+ * lambda (host, R entries) receives the weights (GetFinalLambda).  Per
+ * iteration and mode: MTTKRP (float32 or bfloat16 factor copies per
+ * mttkrp_dtype), float64 Hadamard of the other Grams, float64 Cholesky
+ * inverse (LU fallback), float64 row solve, column 2-norm (iteration 0) or
+ * max-norm, MFMA Gram of the new factor.  The fit (float64) is
+ * 1 - |X - [[lambda; U]]|_F / |X|_F; the loop stops after max_iters or when
+ * the fit changes by less than tol.
+ */
+int mmcsf_cpd_als(const mmcsf_format *f, int R, int max_iters, double tol,
+                  int mttkrp_dtype, float *const *factors, double *lambda,
+                  mmcsf_cpd_stats *stats, void *stream);
+
+/* G = U^T U (float64 R x R, device) of a device float32 rows x R matrix, with
+ * MFMA (v_mfma_f32_16x16x4f32) over row chunks and float64 chunk sums
+ * (the ssyrk_ calls of cpd_cpu.cu:543-546, 649-652). */
+int mmcsf_gram(const float *U, uint64_t rows, int R, double *G, void *stream);
 
 /* ---- version ---------------------------------------------------------------- */
 const char *mmcsf_version(void);

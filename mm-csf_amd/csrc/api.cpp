@@ -48,15 +48,6 @@ void check_shape(int ndims, const uint64_t *dims, uint64_t nnz) {
     MMCSF_REQUIRE(nnz <= kMaxIndex, MMCSF_ERR_UNSUPPORTED, "nnz must be below 2^31");
 }
 
-__global__ void k_u64_to_u32_check(const uint64_t *in, uint32_t *out, uint64_t n, uint64_t lim,
-                                   uint32_t *bad) {
-    uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    uint64_t v = in[i];
-    if (v >= lim) atomicOr(bad, 1u);
-    out[i] = (uint32_t)v;
-}
-
 __global__ void k_u32_check(const uint32_t *in, uint64_t n, uint64_t lim, uint32_t *bad) {
     uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
     if (i < n && in[i] >= lim) atomicOr(bad, 1u);
@@ -152,6 +143,7 @@ int mmcsf_options_default(mmcsf_options *o) {
     o->natOrdering = 0;
     o->fbrThreashold = 99999999;
     o->factorDtype = MMCSF_F32;
+    o->seed = 1;
     std::strcpy(o->m0, "012");
     std::strcpy(o->m1, "120");
     std::strcpy(o->m2, "201");
@@ -184,6 +176,8 @@ int mmcsf_options_parse(int argc, char **argv, mmcsf_options *o) {
                 case 'v': o->verbose = std::atoi(v) == 1; break;
                 case 'c': o->correctness = std::atoi(v) == 1; break;
                 case 'd': o->factorDtype = std::atoi(v) == 1 ? MMCSF_BF16 : MMCSF_F32; break;
+                case 'n': o->cpdIters = as_u64(v); break;
+                case 'x': o->seed = std::strtoull(v, nullptr, 0); break;
                 case 'i':
                     MMCSF_REQUIRE(std::strlen(v) < sizeof(o->inFileName), MMCSF_ERR_ARG, "input path too long");
                     std::strcpy(o->inFileName, v);
@@ -630,6 +624,30 @@ int mmcsf_mttkrp_coo_host(const mmcsf_format *fh, int mode, int R, const float *
                 }
             }
         }
+    });
+}
+
+int mmcsf_cpd_als(const mmcsf_format *fh, int R, int max_iters, double tol, int dtype, float *const *factors,
+                  double *lambda, mmcsf_cpd_stats *stats, void *stream) {
+    return guarded([&] {
+        MMCSF_REQUIRE(fh && factors, MMCSF_ERR_ARG, "null argument");
+        const Format &f = fh->f;
+        MMCSF_REQUIRE(f.type == MMCSF_TYPE_MMCSF, MMCSF_ERR_ARG, "CP-ALS runs on an MM-CSF format");
+        MMCSF_REQUIRE(R >= 1 && R <= 256, MMCSF_ERR_UNSUPPORTED, "R must be in 1..256");
+        MMCSF_REQUIRE(dtype == MMCSF_F32 || dtype == MMCSF_BF16, MMCSF_ERR_ARG, "dtype must be F32 or BF16");
+        MMCSF_REQUIRE(f.nnz > 0, MMCSF_ERR_ARG, "CP-ALS needs at least one nonzero");
+        MMCSF_REQUIRE(max_iters >= 1, MMCSF_ERR_ARG, "max_iters must be positive");
+        for (int m = 0; m < f.ndims; ++m) MMCSF_REQUIRE(factors[m] != nullptr, MMCSF_ERR_ARG, "missing factor");
+        if (stats) std::memset(stats, 0, sizeof(*stats));
+        cpd_als(f, R, max_iters, tol, dtype, factors, lambda, stats, static_cast<hipStream_t>(stream));
+    });
+}
+
+int mmcsf_gram(const float *U, uint64_t rows, int R, double *G, void *stream) {
+    return guarded([&] {
+        MMCSF_REQUIRE(U && G, MMCSF_ERR_ARG, "null argument");
+        MMCSF_REQUIRE(R >= 1 && R <= 256 && rows <= kMaxIndex, MMCSF_ERR_UNSUPPORTED, "unsupported shape");
+        gram(U, rows, R, G, static_cast<hipStream_t>(stream));
     });
 }
 

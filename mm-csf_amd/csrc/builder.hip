@@ -199,11 +199,6 @@ __global__ void k_scatter_upper(const uint32_t *__restrict__ flag, const uint32_
 
 __global__ void k_set(uint32_t *p, uint64_t i, uint32_t v) { p[i] = v; }
 
-__global__ void k_fill_u32(uint32_t *p, uint64_t n, uint32_t v) {
-    uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
-    if (i < n) p[i] = v;
-}
-
 // ---------------------------------------------------------------------------
 // bins
 // ---------------------------------------------------------------------------

@@ -1,0 +1,451 @@
+// CP-ALS on the GPU (cpd() of the reference, src/cpd_cpu.cu:484-691).
+//
+// Everything stays on the device; the host reads one float64 (the fit) per
+// iteration.  Arithmetic per step:
+//   MTTKRP                 the MM-CSF kernels (float32 accumulation)
+//   Gram  A^T A            MFMA v_mfma_f32_16x16x4f32 over row chunks, chunk sums
+//                          reduced in float64
+//   Hadamard of Grams, R x R Cholesky (LU with partial pivoting when the
+//   matrix is not SPD), inverse, the row solve U = M H^-1, column norms and
+//   the fit                float64
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "mmcsf_internal.hpp"
+
+namespace mmcsf {
+
+namespace {
+
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+// ---------------------------------------------------------------------------
+// Gram: G += U^T U for U row-major rows x R.  blockIdx.y picks a 16x16 tile
+// pair (bi <= bj); every wave reduces `rowsPerWave` rows with MFMA:
+//   A operand (16x4)  lane l: U[k0 + l/16][16*bi + l%16]
+//   B operand (4x16)  lane l: U[k0 + l/16][16*bj + l%16]
+//   D (16x16)         lane l, reg r: D[4*(l/16) + r][l%16]
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_gram(const float *__restrict__ U, uint32_t rows, int R,
+                                              uint32_t rowsPerWave, double *__restrict__ G) {
+    const int nb = (R + 15) / 16;
+    int p = blockIdx.y, bi = 0;
+    while (p >= nb - bi) {  // tile pair p -> (bi, bj), bi <= bj
+        p -= nb - bi;
+        ++bi;
+    }
+    const int bj = bi + p;
+    const int lane = threadIdx.x & 63;
+    const uint64_t wave = (uint64_t)blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+    const uint64_t r0 = wave * rowsPerWave;
+    if (r0 >= rows) return;
+    const uint64_t r1 = min((uint64_t)rows, r0 + rowsPerWave);
+    const int ci = 16 * bi + (lane & 15), cj = 16 * bj + (lane & 15);
+    const int kk = lane >> 4;
+    v4f acc = {0.f, 0.f, 0.f, 0.f};
+    for (uint64_t k = r0; k < r1; k += 4) {
+        const uint64_t row = k + kk;
+        float a = 0.f, b = 0.f;
+        if (row < r1) {
+            const float *u = U + row * (uint64_t)R;
+            if (ci < R) a = u[ci];
+            if (cj < R) b = u[cj];
+        }
+        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc, 0, 0, 0);
+    }
+    for (int r = 0; r < 4; ++r) {
+        const int i = 16 * bi + 4 * (lane >> 4) + r;
+        const int j = cj;
+        if (i < R && j < R) {
+            unsafeAtomicAdd(G + (size_t)i * R + j, (double)acc[r]);
+            if (bi != bj) unsafeAtomicAdd(G + (size_t)j * R + i, (double)acc[r]);
+        }
+    }
+}
+
+__global__ void k_zero_d(double *p, uint64_t n) {
+    uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.0;
+}
+
+// ---------------------------------------------------------------------------
+// H = Hadamard_{n != skip} G[n]; Hinv = H^-1 by Cholesky (lower), or by LU
+// with partial pivoting when a pivot is not positive (MatrixSolveNormals,
+// cpd_cpu.cu:140-208).  One workgroup; W is an R*R*3 float64 workspace.
+// ---------------------------------------------------------------------------
+__global__ void k_normal_inverse(const double *__restrict__ G, int N, int skip, int R, double *__restrict__ W,
+                                 double *__restrict__ Hinv, int *__restrict__ status) {
+    const int tid = threadIdx.x, nt = blockDim.x;
+    double *H = W, *L = W + (size_t)R * R, *X = W + 2 * (size_t)R * R;
+    __shared__ int fail;
+    for (int e = tid; e < R * R; e += nt) {
+        double h = 1.0;
+        for (int n = 0; n < N; ++n)
+            if (n != skip) h *= G[(size_t)n * R * R + e];
+        H[e] = h;
+        L[e] = 0.0;
+    }
+    if (tid == 0) fail = 0;
+    __syncthreads();
+    // Cholesky, column by column
+    for (int j = 0; j < R; ++j) {
+        if (tid == 0) {
+            double d = H[(size_t)j * R + j];
+            for (int k = 0; k < j; ++k) d -= L[(size_t)j * R + k] * L[(size_t)j * R + k];
+            if (!(d > 0.0)) fail = 1;
+            L[(size_t)j * R + j] = d > 0.0 ? sqrt(d) : 1.0;
+        }
+        __syncthreads();
+        if (fail) break;
+        for (int i = j + 1 + tid; i < R; i += nt) {
+            double s = H[(size_t)i * R + j];
+            for (int k = 0; k < j; ++k) s -= L[(size_t)i * R + k] * L[(size_t)j * R + k];
+            L[(size_t)i * R + j] = s / L[(size_t)j * R + j];
+        }
+        __syncthreads();
+    }
+    if (!fail) {
+        // X = L^-1 (lower), one column per thread: L x = e_c
+        for (int c = tid; c < R; c += nt) {
+            for (int i = 0; i < R; ++i) {
+                double s = (i == c) ? 1.0 : 0.0;
+                for (int k = c; k < i; ++k) s -= L[(size_t)i * R + k] * X[(size_t)k * R + c];
+                X[(size_t)i * R + c] = i < c ? 0.0 : s / L[(size_t)i * R + i];
+            }
+        }
+        __syncthreads();
+        // H^-1 = X^T X
+        for (int e = tid; e < R * R; e += nt) {
+            const int i = e / R, j = e % R;
+            double s = 0.0;
+            for (int k = max(i, j); k < R; ++k) s += X[(size_t)k * R + i] * X[(size_t)k * R + j];
+            Hinv[e] = s;
+        }
+        if (tid == 0) status[0] = 0;
+        return;
+    }
+    // LU with partial pivoting on a fresh copy of H (one thread: R is small)
+    if (tid == 0) {
+        status[0] = 1;
+        for (int e = 0; e < R * R; ++e) {
+            double h = 1.0;
+            for (int n = 0; n < N; ++n)
+                if (n != skip) h *= G[(size_t)n * R * R + e];
+            L[e] = h;                              // LU in place
+            X[e] = (e / R == e % R) ? 1.0 : 0.0;  // right-hand sides: identity
+        }
+        for (int k = 0; k < R; ++k) {
+            int piv = k;
+            double best = fabs(L[(size_t)k * R + k]);
+            for (int i = k + 1; i < R; ++i)
+                if (fabs(L[(size_t)i * R + k]) > best) { best = fabs(L[(size_t)i * R + k]); piv = i; }
+            if (best == 0.0) { status[0] = 2; break; }
+            if (piv != k)
+                for (int c = 0; c < R; ++c) {
+                    double t = L[(size_t)k * R + c]; L[(size_t)k * R + c] = L[(size_t)piv * R + c]; L[(size_t)piv * R + c] = t;
+                    t = X[(size_t)k * R + c]; X[(size_t)k * R + c] = X[(size_t)piv * R + c]; X[(size_t)piv * R + c] = t;
+                }
+            for (int i = k + 1; i < R; ++i) {
+                double f = L[(size_t)i * R + k] / L[(size_t)k * R + k];
+                for (int c = k; c < R; ++c) L[(size_t)i * R + c] -= f * L[(size_t)k * R + c];
+                for (int c = 0; c < R; ++c) X[(size_t)i * R + c] -= f * X[(size_t)k * R + c];
+            }
+        }
+        if (status[0] == 1)
+            for (int c = 0; c < R; ++c)
+                for (int i = R - 1; i >= 0; --i) {
+                    double s = X[(size_t)i * R + c];
+                    for (int k = i + 1; k < R; ++k) s -= L[(size_t)i * R + k] * Hinv[(size_t)k * R + c];
+                    Hinv[(size_t)i * R + c] = s / L[(size_t)i * R + i];
+                }
+    }
+}
+
+// U[i][j] = sum_k M[i][k] Hinv[k][j] (float64 accumulation)
+__global__ __launch_bounds__(256) void k_apply_inverse(const float *__restrict__ M, const double *__restrict__ Hinv,
+                                                       uint32_t rows, int R, float *__restrict__ U) {
+    extern __shared__ double hs[];
+    const bool inLds = (size_t)R * R * sizeof(double) <= 32768;
+    if (inLds) {
+        for (int e = threadIdx.x; e < R * R; e += blockDim.x) hs[e] = Hinv[e];
+        __syncthreads();
+    }
+    const double *Hp = inLds ? hs : Hinv;
+    const uint64_t total = (uint64_t)rows * R;
+    for (uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < total;
+         e += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t i = e / R;
+        const int j = (int)(e % R);
+        const float *m = M + i * R;
+        double s = 0.0;
+        for (int k = 0; k < R; ++k) s += (double)m[k] * Hp[(size_t)k * R + j];
+        U[e] = (float)s;
+    }
+}
+
+// column 2-norms (kind 0, float64 sums of squares) or maxima (kind 1,
+// largest non-negative entry as an order-preserving uint32 bit pattern)
+__global__ __launch_bounds__(256) void k_colnorm(const float *__restrict__ U, uint32_t rows, int R, int kind,
+                                                 double *__restrict__ sq, uint32_t *__restrict__ mx) {
+    const int j = threadIdx.x % R;
+    const int rowsPerPass = blockDim.x / R;
+    const int sub = threadIdx.x / R;
+    if (sub >= rowsPerPass) return;
+    double s = 0.0;
+    float m = 0.f;
+    for (uint64_t i = (uint64_t)blockIdx.x * rowsPerPass + sub; i < rows; i += (uint64_t)gridDim.x * rowsPerPass) {
+        const float v = U[i * R + j];
+        s += (double)v * v;
+        m = fmaxf(m, v);
+    }
+    if (kind == 0) unsafeAtomicAdd(sq + j, s);
+    else atomicMax(mx + j, __float_as_uint(m));
+}
+
+__global__ void k_lambda(int R, int kind, const double *__restrict__ sq, const uint32_t *__restrict__ mx,
+                         double *__restrict__ lam) {
+    const int j = threadIdx.x;
+    if (j >= R) return;
+    if (kind == 0) lam[j] = sqrt(sq[j]);
+    else lam[j] = fmax((double)__uint_as_float(mx[j]), 1.0);  // MatrixMaxNorm clamp (cpd_cpu.cu:316-319)
+}
+
+__global__ void k_scale_cols(float *__restrict__ U, uint32_t rows, int R, const double *__restrict__ lam,
+                             double *__restrict__ lamAcc) {
+    const uint64_t total = (uint64_t)rows * R;
+    for (uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < total;
+         e += (uint64_t)gridDim.x * blockDim.x) {
+        const double l = lam[e % R];
+        U[e] = (float)((double)U[e] / (l != 0.0 ? l : 1.0));
+    }
+    if (lamAcc && blockIdx.x == 0 && threadIdx.x < R) lamAcc[threadIdx.x] *= lam[threadIdx.x];
+}
+
+__global__ __launch_bounds__(256) void k_dot_lambda(const float *__restrict__ A, const float *__restrict__ B,
+                                                    uint32_t rows, int R, const double *__restrict__ lam,
+                                                    double *__restrict__ acc) {
+    double s = 0.0;
+    const uint64_t total = (uint64_t)rows * R;
+    for (uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < total;
+         e += (uint64_t)gridDim.x * blockDim.x)
+        s += (double)A[e] * (double)B[e] * lam[e % R];
+    __shared__ double red[256];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+        if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) unsafeAtomicAdd(acc, red[0]);
+}
+
+__global__ __launch_bounds__(256) void k_sumsq(const float *__restrict__ v, uint64_t n, double *__restrict__ acc) {
+    double s = 0.0;
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        s += (double)v[i] * v[i];
+    __shared__ double red[256];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+        if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) unsafeAtomicAdd(acc, red[0]);
+}
+
+// fit = 1 - sqrt(|X|^2 + lam^T (Hadamard_m G_m) lam - 2 inner) / |X|
+// (KruskalTensorFit, cpd_cpu.cu:440-465, with the full symmetric product)
+__global__ void k_fit(const double *__restrict__ G, int N, int R, const double *__restrict__ lam,
+                      const double *__restrict__ inner, const double *__restrict__ xnormsq, double *__restrict__ fit) {
+    __shared__ double red[256];
+    double s = 0.0;
+    for (int e = threadIdx.x; e < R * R; e += blockDim.x) {
+        double h = 1.0;
+        for (int n = 0; n < N; ++n) h *= G[(size_t)n * R * R + e];
+        s += h * lam[e / R] * lam[e % R];
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+        if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        double normMats = fabs(red[0]);
+        double resid = xnormsq[0] + normMats - 2.0 * inner[0];
+        if (resid > 0.0) resid = sqrt(resid);
+        fit[0] = 1.0 - resid / sqrt(xnormsq[0]);
+    }
+}
+
+__global__ void k_f32_to_bf16(const float *__restrict__ in, uint16_t *__restrict__ out, uint64_t n) {
+    uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t u = __float_as_uint(in[i]);
+    u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even
+    out[i] = (uint16_t)(u >> 16);
+}
+
+__global__ void k_fill_d(double *p, int n, double v) {
+    int i = threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+unsigned grid_for(uint64_t n, unsigned cap = 65536) {
+    return (unsigned)std::max<uint64_t>(1, std::min<uint64_t>((n + 255) / 256, cap));
+}
+
+unsigned colnorm_grid(uint64_t rows, int R) {
+    const uint64_t perPass = std::max(1, 256 / R);
+    return (unsigned)std::max<uint64_t>(1, std::min<uint64_t>((rows + perPass - 1) / perPass, 2048));
+}
+
+}  // namespace
+
+void gram(const float *U, uint64_t rows, int R, double *G, hipStream_t s) {
+    hipLaunchKernelGGL(k_zero_d, dim3(grid_for((uint64_t)R * R)), dim3(256), 0, s, G, (uint64_t)R * R);
+    if (rows == 0) return;
+    const int nb = (R + 15) / 16;
+    const uint32_t rowsPerWave = 1024;
+    const uint64_t waves = (rows + rowsPerWave - 1) / rowsPerWave;
+    dim3 grid((unsigned)((waves + 3) / 4), (unsigned)(nb * (nb + 1) / 2));
+    hipLaunchKernelGGL(k_gram, grid, dim3(256), 0, s, U, (uint32_t)rows, R, rowsPerWave, G);
+    MMCSF_HIP(hipGetLastError());
+}
+
+// CP-ALS driver state
+struct CpdRun {
+    const Format &f;
+    int N, R;
+    hipStream_t s;
+    DevArray<double> G, W, Hinv, lam, lamFinal, sq, scal;
+    DevArray<uint32_t> mx;
+    DevArray<int> status;
+    DevArray<float> M;
+    std::vector<DevArray<uint16_t>> Ub;
+    CpdRun(const Format &fmt, int R_, hipStream_t st) : f(fmt), N(fmt.ndims), R(R_), s(st) {}
+};
+
+void cpd_als(const Format &f, int R, int iters, double tol, int dtype, float *const *U, double *lambdaHost,
+             mmcsf_cpd_stats *stats, hipStream_t s) {
+    const int N = f.ndims;
+    CpdRun st(f, R, s);
+    const size_t RR = (size_t)R * R;
+    uint64_t maxRows = 0;
+    for (int m = 0; m < N; ++m) maxRows = std::max<uint64_t>(maxRows, f.dims[m]);
+    st.G.alloc(N * RR);
+    st.W.alloc(3 * RR);
+    st.Hinv.alloc(RR);
+    st.lam.alloc(R);
+    st.lamFinal.alloc(R);
+    st.sq.alloc(R);
+    st.mx.alloc(R);
+    st.scal.alloc(4);  // [0] |X|^2, [1] inner, [2] fit
+    st.status.alloc(1);
+    st.M.alloc(maxRows * R);
+    if (dtype == MMCSF_BF16) {
+        st.Ub.resize(N);
+        for (int m = 0; m < N; ++m) st.Ub[m].alloc(f.dims[m] * R);
+    }
+    auto refresh_bf16 = [&](int m) {
+        if (dtype == MMCSF_BF16)
+            hipLaunchKernelGGL(k_f32_to_bf16, dim3(grid_for(f.dims[m] * R, 1u << 30)), dim3(256), 0, s, U[m],
+                               st.Ub[m].p, (uint64_t)f.dims[m] * R);
+    };
+    for (int m = 0; m < N; ++m) {
+        gram(U[m], f.dims[m], R, st.G.p + m * RR, s);
+        refresh_bf16(m);
+    }
+    hipLaunchKernelGGL(k_zero_d, dim3(1), dim3(256), 0, s, st.scal.p, (uint64_t)4);
+    hipLaunchKernelGGL(k_sumsq, dim3(grid_for(f.nnz, 4096)), dim3(256), 0, s, f.csf.vals.p, (uint64_t)f.nnz,
+                       st.scal.p);
+    hipLaunchKernelGGL(k_fill_d, dim3(1), dim3(256), 0, s, st.lam.p, R, 1.0);
+    Factors F{};
+    F.dtype = dtype;
+    F.R = R;
+    for (int m = 0; m < N; ++m) F.U[m] = dtype == MMCSF_BF16 ? (const void *)st.Ub[m].p : (const void *)U[m];
+    hipEvent_t e0, e1;
+    MMCSF_HIP(hipEventCreate(&e0));
+    MMCSF_HIP(hipEventCreate(&e1));
+    double mttkrpMs = 0, oldfit = 0, fit = 0;
+    int it = 0, lu = 0;
+    auto t0 = std::chrono::steady_clock::now();
+    for (it = 0; it < iters; ++it) {
+        for (int m = 0; m < N; ++m) {
+            int lvl = 0;
+            for (int l = 0; l < N; ++l)
+                if (f.csf.order[l] == m) lvl = l;
+            MMCSF_HIP(hipEventRecord(e0, s));
+            mttkrp_csf(f.csf, f.dims, lvl, F, st.M.p, s);
+            MMCSF_HIP(hipEventRecord(e1, s));
+            hipLaunchKernelGGL(k_normal_inverse, dim3(1), dim3(256), 0, s, st.G.p, N, m, R, st.W.p, st.Hinv.p,
+                               st.status.p);
+            const size_t shm = RR * sizeof(double) <= 32768 ? RR * sizeof(double) : 0;
+            hipLaunchKernelGGL(k_apply_inverse, dim3(grid_for(f.dims[m] * R, 8192)), dim3(256), shm, s, st.M.p,
+                               st.Hinv.p, (uint32_t)f.dims[m], R, U[m]);
+            const int kind = it == 0 ? 0 : 1;  // 2-norm at iteration 0, max-norm after
+            MMCSF_HIP(hipMemsetAsync(st.sq.p, 0, R * sizeof(double), s));
+            MMCSF_HIP(hipMemsetAsync(st.mx.p, 0, R * sizeof(uint32_t), s));
+            hipLaunchKernelGGL(k_colnorm, dim3(colnorm_grid(f.dims[m], R)), dim3(256), 0, s, U[m],
+                               (uint32_t)f.dims[m], R, kind, st.sq.p, st.mx.p);
+            hipLaunchKernelGGL(k_lambda, dim3(1), dim3(std::max(R, 64)), 0, s, R, kind, st.sq.p, st.mx.p, st.lam.p);
+            hipLaunchKernelGGL(k_scale_cols, dim3(grid_for(f.dims[m] * R, 8192)), dim3(256), 0, s, U[m],
+                               (uint32_t)f.dims[m], R, st.lam.p, (double *)nullptr);
+            gram(U[m], f.dims[m], R, st.G.p + m * RR, s);
+            refresh_bf16(m);
+            if (m == N - 1) {
+                // inner product of the last MTTKRP with the updated last factor
+                MMCSF_HIP(hipMemsetAsync(st.scal.p + 1, 0, sizeof(double), s));
+                hipLaunchKernelGGL(k_dot_lambda, dim3(grid_for(f.dims[m] * R, 4096)), dim3(256), 0, s, U[m], st.M.p,
+                                   (uint32_t)f.dims[m], R, st.lam.p, st.scal.p + 1);
+            }
+            int stat = 0;
+            MMCSF_HIP(hipMemcpyAsync(&stat, st.status.p, sizeof(int), hipMemcpyDeviceToHost, s));
+            MMCSF_HIP(hipEventSynchronize(e1));
+            float ms = 0;
+            MMCSF_HIP(hipEventElapsedTime(&ms, e0, e1));
+            mttkrpMs += ms;
+            MMCSF_HIP(hipStreamSynchronize(s));
+            if (stat == 1) ++lu;
+            MMCSF_REQUIRE(stat != 2, MMCSF_ERR_UNSUPPORTED, "normal equations are singular");
+        }
+        hipLaunchKernelGGL(k_fit, dim3(1), dim3(256), 0, s, st.G.p, N, R, st.lam.p, st.scal.p + 1, st.scal.p,
+                           st.scal.p + 2);
+        MMCSF_HIP(hipMemcpyAsync(&fit, st.scal.p + 2, sizeof(double), hipMemcpyDeviceToHost, s));
+        MMCSF_HIP(hipStreamSynchronize(s));
+        if (stats && it < (int)(sizeof(stats->fit_history) / sizeof(double))) stats->fit_history[it] = fit;
+        if (it > 0 && std::fabs(fit - oldfit) < tol) {  // cpd_cpu.cu:670-672
+            ++it;
+            break;
+        }
+        oldfit = fit;
+    }
+    auto t1 = std::chrono::steady_clock::now();
+    // GetFinalLambda (cpd_cpu.cu:333-349): fold the column 2-norms into lambda
+    MMCSF_HIP(hipMemcpyAsync(st.lamFinal.p, st.lam.p, R * sizeof(double), hipMemcpyDeviceToDevice, s));
+    for (int m = 0; m < N; ++m) {
+        MMCSF_HIP(hipMemsetAsync(st.sq.p, 0, R * sizeof(double), s));
+        hipLaunchKernelGGL(k_colnorm, dim3(colnorm_grid(f.dims[m], R)), dim3(256), 0, s, U[m],
+                           (uint32_t)f.dims[m], R, 0, st.sq.p, st.mx.p);
+        hipLaunchKernelGGL(k_lambda, dim3(1), dim3(std::max(R, 64)), 0, s, R, 0, st.sq.p, st.mx.p, st.lam.p);
+        hipLaunchKernelGGL(k_scale_cols, dim3(grid_for(f.dims[m] * R, 8192)), dim3(256), 0, s, U[m],
+                           (uint32_t)f.dims[m], R, st.lam.p, st.lamFinal.p);
+    }
+    if (lambdaHost)
+        MMCSF_HIP(hipMemcpyAsync(lambdaHost, st.lamFinal.p, R * sizeof(double), hipMemcpyDeviceToHost, s));
+    MMCSF_HIP(hipStreamSynchronize(s));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (stats) {
+        stats->iterations = it;
+        stats->fit = fit;
+        stats->ms_total = std::chrono::duration<double, std::milli>(t1 - t0).count();
+        stats->ms_mttkrp = mttkrpMs;
+        stats->lu_fallbacks = lu;
+    }
+}
+
+}  // namespace mmcsf

@@ -185,6 +185,11 @@ void mttkrp_coo(const Format &f, const Factors &F, float *out, hipStream_t s);
 void mttkrp_hyb(const Hyb &h, const uint64_t *dims, const Factors &F, float *out, hipStream_t s);
 void fill_zero(float *p, uint64_t n, hipStream_t s);
 
+// CP-ALS (cpd.hip)
+void gram(const float *U, uint64_t rows, int R, double *G, hipStream_t s);
+void cpd_als(const Format &f, int R, int iters, double tol, int dtype, float *const *U, double *lambdaHost,
+             mmcsf_cpd_stats *stats, hipStream_t s);
+
 // generator (generate.hip)
 void generate_tensor(int ndims, const uint64_t *dims, uint64_t nnz, const int *law,
                      const double *alpha, uint64_t seed, Tensor &t, hipStream_t s);

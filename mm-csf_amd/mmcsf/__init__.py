@@ -19,7 +19,7 @@ import numpy as np
 __all__ = [
     "lib", "MMCSFError", "Options", "Tensor", "Format", "mttkrp", "mttkrp_allmode",
     "TYPE_COO_CPU", "TYPE_COO", "TYPE_BCSF", "TYPE_HBCSF", "TYPE_MMCSF", "F32", "BF16",
-    "parse_options", "to_bf16",
+    "parse_options", "to_bf16", "cpd_als", "gram",
 ]
 
 TYPE_COO_CPU, TYPE_COO, TYPE_BCSF, TYPE_HBCSF, TYPE_MMCSF = 1, 3, 8, 10, 12
@@ -42,7 +42,7 @@ class Options(C.Structure):
         ("TBsize", C.c_uint64), ("MIfbTh", C.c_uint64), ("fiberPerWarp", C.c_uint64),
         ("verbose", C.c_int), ("correctness", C.c_int), ("nBin", C.c_uint64),
         ("cpdIters", C.c_uint64), ("natOrdering", C.c_int), ("fbrThreashold", C.c_uint64),
-        ("factorDtype", C.c_int), ("inFileName", C.c_char * 4096), ("outFileName", C.c_char * 4096),
+        ("factorDtype", C.c_int), ("seed", C.c_uint64), ("inFileName", C.c_char * 4096), ("outFileName", C.c_char * 4096),
         ("m0", C.c_char * 16), ("m1", C.c_char * 16), ("m2", C.c_char * 16),
     ]
 
@@ -53,6 +53,13 @@ class FormatInfo(C.Structure):
         ("modeOrder", C.c_int * 4), ("nLevel", C.c_uint64 * 4), ("nFibers", C.c_uint64),
         ("droppedSlices", C.c_uint64), ("COOnnz", C.c_uint64), ("CSLnnz", C.c_uint64),
         ("HCSRnnz", C.c_uint64), ("redunMode", C.c_int),
+    ]
+
+
+class CpdStats(C.Structure):
+    _fields_ = [
+        ("iterations", C.c_int), ("fit", C.c_double), ("fit_history", C.c_double * 64),
+        ("ms_total", C.c_double), ("ms_mttkrp", C.c_double), ("lu_fallbacks", C.c_int),
     ]
 
 
@@ -94,6 +101,9 @@ def lib():
     L.mmcsf_mttkrp.argtypes = [VP, C.c_int, C.c_int, C.c_int, C.POINTER(VP), VP, VP]
     L.mmcsf_mttkrp_allmode.argtypes = [VP, C.c_int, C.c_int, C.POINTER(VP), C.POINTER(VP), VP,
                                        C.POINTER(C.c_float)]
+    L.mmcsf_cpd_als.argtypes = [VP, C.c_int, C.c_int, C.c_double, C.c_int, C.POINTER(VP),
+                                C.POINTER(C.c_double), C.POINTER(CpdStats), VP]
+    L.mmcsf_gram.argtypes = [VP, C.c_uint64, C.c_int, VP, VP]
     L.mmcsf_mttkrp_coo_host.argtypes = [VP, C.c_int, C.c_int, C.POINTER(C.POINTER(C.c_float)),
                                         C.POINTER(C.c_float)]
     _LIB = L
@@ -316,3 +326,33 @@ def mttkrp_allmode(fmt, factors, outs=None, stream=None, timed=False):
     _check(lib().mmcsf_mttkrp_allmode(fmt._h, int(R), _dtype_of(factors), _factor_ptrs(factors), op,
                                       _stream_ptr(stream), ms))
     return outs, ([float(v) for v in ms] if timed else None)
+
+
+def gram(U, stream=None):
+    """U^T U (float64) of a device float32 matrix, via MFMA."""
+    import torch
+    assert U.is_cuda and U.dtype == torch.float32 and U.is_contiguous() and U.dim() == 2
+    R = U.shape[1]
+    G = torch.empty((R, R), dtype=torch.float64, device=U.device)
+    _check(lib().mmcsf_gram(VP(U.data_ptr()), U.shape[0], R, VP(G.data_ptr()), _stream_ptr(stream)))
+    return G
+
+
+def cpd_als(fmt, factors, iters=10, tol=1e-5, mttkrp_bf16=False, stream=None):
+    """CP-ALS (cpd(), cpd_cpu.cu:484-691) on an MM-CSF format.
+
+    ``factors``: device float32 tensors (dims[m], R), updated in place to the
+    final normalised factors.  Returns (lambda, stats dict)."""
+    info = fmt.info()
+    R = factors[0].shape[1]
+    _check_factors(info["dims"], factors, R)
+    fp = (VP * len(factors))(*[VP(f.data_ptr()) for f in factors])
+    lam = (C.c_double * R)()
+    st = CpdStats()
+    _check(lib().mmcsf_cpd_als(fmt._h, int(R), int(iters), float(tol), BF16 if mttkrp_bf16 else F32, fp, lam,
+                               C.byref(st), _stream_ptr(stream)))
+    n = st.iterations
+    return np.array(lam[:], dtype=np.float64), {
+        "iterations": n, "fit": st.fit, "fit_history": list(st.fit_history[: min(n, 64)]),
+        "ms_total": st.ms_total, "ms_mttkrp": st.ms_mttkrp, "lu_fallbacks": st.lu_fallbacks,
+    }

@@ -438,3 +438,57 @@ def test_hyb_reconstructs_sorted_coo(dims):
     assert h["COOnnz"] + h["CSLnnz"] + h["HCSRnnz"] == len(vals)
     if N == 4:
         assert h["CSLnnz"] == 0
+
+
+# ---------------------------------------------------------------------------
+# float64 CP-ALS and CP-completion restatements (oracle/dense.py)
+# ---------------------------------------------------------------------------
+def _dense_model(U, lam):
+    letters = "ijkl"[: len(U)]
+    spec = ",".join(c + "r" for c in letters) + ",r->" + letters
+    return np.einsum(spec, *U, lam)
+
+
+def test_cpd_oracle_fit_matches_dense_residual():
+    from oracle import dense as D
+    rng = np.random.default_rng(4)
+    dims = [12, 10, 8]
+    inds, vals = random_tensor(rng, dims, 500)
+    U0 = [rng.random((d, 3)) for d in dims]
+    fits, U, lam = D.cpd_als(inds, vals, dims, U0, iters=4, tol=0.0)
+    X = dense_of(dims, inds, vals)
+    model = _dense_model(U, lam)
+    fit_dense = 1 - np.linalg.norm(X - model) / np.linalg.norm(X)
+    np.testing.assert_allclose(fits[-1], fit_dense, rtol=1e-10, atol=1e-12)
+
+
+def test_cpd_oracle_recovers_planted_rank():
+    from oracle import dense as D
+    rng = np.random.default_rng(8)
+    dims = [9, 8, 7]
+    A = [rng.standard_normal((d, 2)) for d in dims]
+    X = _dense_model(A, np.ones(2))
+    inds = [a.ravel() for a in np.indices(dims)]
+    vals = X.ravel().astype(np.float32)
+    fits, U, lam = D.cpd_als(inds, vals, dims, [rng.random((d, 2)) for d in dims], iters=200, tol=1e-12)
+    assert fits[-1] > 0.99999
+    # ALS never decreases the fit (beyond round-off) once the iterations use max-norm scaling
+    assert all(b >= a - 1e-9 for a, b in zip(fits[1:], fits[2:]))
+
+
+def test_cp_grad_matches_finite_differences():
+    from oracle import dense as D
+    rng = np.random.default_rng(9)
+    dims = [6, 5, 4]
+    inds, vals = random_tensor(rng, dims, 60)
+    U = [rng.standard_normal((d, 3)) for d in dims]
+    g = D.cp_grad(inds, vals, U)
+    h = 1e-6
+    for m in range(3):
+        for (i, r) in [(0, 0), (2, 1), (dims[m] - 1, 2)]:
+            Up = [u.copy() for u in U]
+            Um = [u.copy() for u in U]
+            Up[m][i, r] += h
+            Um[m][i, r] -= h
+            fd = (D.cp_loss(inds, vals, Up)[0] - D.cp_loss(inds, vals, Um)[0]) / (2 * h)
+            np.testing.assert_allclose(g[m][i, r], fd, rtol=1e-6, atol=1e-8)
