@@ -196,6 +196,21 @@ def main():
     b16_out = [o.clone() for o in outs]
     log(f"bfloat16: {b16_ms:.3f} ms/step, kernels {b16_k}")
 
+    # the reference's layout: one MM-CSF tree, non-root modes scatter (LDS
+    # privatisation of hot rows, atomics for the rest)
+    log("timing the single-tree path")
+    fmt1 = mmcsf.Format(X, mmcsf.TYPE_MMCSF, fbr_threshold=128, mode_trees=False)
+    single_steps = []
+    for _ in range(args.warmup):
+        mmcsf.mttkrp_allmode(fmt1, U, outs)
+    for _ in range(max(3, args.steps // 4)):
+        _, ms = mmcsf.mttkrp_allmode(fmt1, U, outs, timed=True)
+        single_steps.append(ms)
+    single_k = [sum(s[m] for s in single_steps) / len(single_steps) for m in range(3)]
+    del fmt1
+    torch.cuda.empty_cache()
+    log(f"single tree: kernels {single_k}")
+
     log("float64 reference")
     err32, err16 = [], []
     for m in range(3):
@@ -255,6 +270,12 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
+        },
+        "single_tree": {
+            "note": "the reference's MM-CSF layout: one tree, modes other than its root scatter",
+            "kernel_ms": [round(k, 4) for k in single_k],
+            "all_mode_kernel_ms": round(sum(single_k), 4),
+            "gbs": round(sum(bytes_f32) / (sum(single_k) * 1e-3) / 1e9, 2),
         },
         "generate_s": round(gen_s, 2),
         "build_s": round(build_s, 2),

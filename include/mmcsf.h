@@ -66,6 +66,9 @@ typedef struct mmcsf_options {
     uint64_t fbrThreashold;   /* -f fiber split threshold (99999999) */
     int factorDtype;          /* -d 0|1: FP32 / BF16 factor storage (new flag) */
     uint64_t seed;            /* -x seed of generated inputs (new flag, 1) */
+    int modeTrees;            /* -a 1|0 (new flag, 1): MM-CSF also keeps a root tree per
+                                 other mode, so every mode's MTTKRP is a root traversal;
+                                 0 = single tree, non-root modes scatter (the reference) */
     char inFileName[4096];    /* -i */
     char outFileName[4096];   /* -o */
     char m0[16], m1[16], m2[16];  /* -p/-q/-r, unused like the reference */
@@ -161,6 +164,7 @@ typedef struct mmcsf_format_info {
     uint64_t droppedSlices;    /* slices the reference's binning drops */
     uint64_t COOnnz, CSLnnz, HCSRnnz;  /* HB-CSF parts */
     int redunMode;             /* mttkrp.cu:315-347 */
+    int modeTrees;             /* internal root trees held besides the MM-CSF tree */
 } mmcsf_format_info;
 
 int mmcsf_format_info_get(const mmcsf_format *f, mmcsf_format_info *info);

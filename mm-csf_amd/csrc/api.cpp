@@ -144,6 +144,7 @@ int mmcsf_options_default(mmcsf_options *o) {
     o->fbrThreashold = 99999999;
     o->factorDtype = MMCSF_F32;
     o->seed = 1;
+    o->modeTrees = 1;
     std::strcpy(o->m0, "012");
     std::strcpy(o->m1, "120");
     std::strcpy(o->m2, "201");
@@ -177,6 +178,7 @@ int mmcsf_options_parse(int argc, char **argv, mmcsf_options *o) {
                 case 'c': o->correctness = std::atoi(v) == 1; break;
                 case 'd': o->factorDtype = std::atoi(v) == 1 ? MMCSF_BF16 : MMCSF_F32; break;
                 case 'n': o->cpdIters = as_u64(v); break;
+                case 'a': o->modeTrees = std::atoi(v) != 0; break;
                 case 'x': o->seed = std::strtoull(v, nullptr, 0); break;
                 case 'i':
                     MMCSF_REQUIRE(std::strlen(v) < sizeof(o->inFileName), MMCSF_ERR_ARG, "input path too long");
@@ -378,8 +380,23 @@ int mmcsf_format_build(const mmcsf_tensor *th, int type, const mmcsf_options *op
             sort_coo(t, order, keys, vals, s);
             uint64_t fs = t.ndims == 4 ? 128 : kNoSplit;  // util.h:1325-1326
             build_csf(t.ndims, t.dims, t.nnz, order, keys, std::move(vals), opt->fbrThreashold, fs, f.csf, s);
-            // privatisation tables of the scatter levels (every level but the root)
-            for (int l = 1; l < t.ndims; ++l) build_hot_table(f.csf, l, t.dims[order[l]], f.csf.hot[l], s);
+            if (opt->modeTrees) {
+                // a root tree for every other mode: each MTTKRP becomes a root traversal
+                for (int m = 0; m < t.ndims; ++m) {
+                    if (m == order[0]) continue;
+                    int o2[kMaxDims];
+                    DevArray<uint32_t> k2[kMaxDims];
+                    DevArray<float> v2;
+                    order_tensormode(t.ndims, t.dims, m, nat, o2);
+                    sort_coo(t, o2, k2, v2, s);
+                    build_csf(t.ndims, t.dims, t.nnz, o2, k2, std::move(v2), opt->fbrThreashold, kNoSplit, f.tree[m],
+                              s);
+                    f.hasTree[m] = true;
+                }
+            } else {
+                // privatisation tables of the scatter levels (every level but the root)
+                for (int l = 1; l < t.ndims; ++l) build_hot_table(f.csf, l, t.dims[order[l]], f.csf.hot[l], s);
+            }
         } else {
             order_tensormode(t.ndims, t.dims, (int)opt->mode, nat, order);
             if (type == MMCSF_TYPE_COO || type == MMCSF_TYPE_COO_CPU) {
@@ -432,6 +449,7 @@ int mmcsf_format_info_get(const mmcsf_format *fh, mmcsf_format_info *info) {
             info->nFibers = (f.type == MMCSF_TYPE_HBCSF ? c->cnt[f.ndims - 2] : c->cnt[1]) + 1;
             info->droppedSlices = c->dropped;
         }
+        for (int m = 0; m < f.ndims; ++m) info->modeTrees += f.hasTree[m] ? 1 : 0;
     });
 }
 
@@ -548,7 +566,7 @@ int mmcsf_mttkrp(const mmcsf_format *fh, int mode, int R, int dtype, const void 
                 mttkrp_hyb(f.hyb, f.dims, F, out, s);
                 break;
             default:
-                mttkrp_csf(f.csf, f.dims, level_of(f.csf.order, f.ndims, mode), F, out, s);
+                mttkrp_mode(f, mode, F, out, s);
                 break;
         }
     });
@@ -580,8 +598,8 @@ int mmcsf_mttkrp_allmode(const mmcsf_format *fh, int R, int dtype, const void *c
         F.R = R;
         for (int m = 0; m < f.ndims; ++m) {
             MMCSF_REQUIRE(outs[m] != nullptr, MMCSF_ERR_ARG, "missing output");
-            mttkrp_csf(f.csf, f.dims, level_of(f.csf.order, f.ndims, m), F, outs[m], s,
-                       ms_per_mode ? ev[2 * m] : nullptr, ms_per_mode ? ev[2 * m + 1] : nullptr);
+            mttkrp_mode(f, m, F, outs[m], s, ms_per_mode ? ev[2 * m] : nullptr,
+                        ms_per_mode ? ev[2 * m + 1] : nullptr);
         }
         if (ms_per_mode) {
             MMCSF_HIP(hipEventSynchronize(ev[2 * f.ndims - 1]));

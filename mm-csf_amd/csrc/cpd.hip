@@ -375,12 +375,7 @@ void cpd_als(const Format &f, int R, int iters, double tol, int dtype, float *co
     auto t0 = std::chrono::steady_clock::now();
     for (it = 0; it < iters; ++it) {
         for (int m = 0; m < N; ++m) {
-            int lvl = 0;
-            for (int l = 0; l < N; ++l)
-                if (f.csf.order[l] == m) lvl = l;
-            MMCSF_HIP(hipEventRecord(e0, s));
-            mttkrp_csf(f.csf, f.dims, lvl, F, st.M.p, s);
-            MMCSF_HIP(hipEventRecord(e1, s));
+            mttkrp_mode(f, m, F, st.M.p, s, e0, e1);
             hipLaunchKernelGGL(k_normal_inverse, dim3(1), dim3(256), 0, s, st.G.p, N, m, R, st.W.p, st.Hinv.p,
                                st.status.p);
             const size_t shm = RR * sizeof(double) <= 32768 ? RR * sizeof(double) : 0;

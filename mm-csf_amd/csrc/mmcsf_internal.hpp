@@ -145,6 +145,10 @@ struct Format {
     DevArray<float> cooVals;
     // types 8 / 12
     Csf csf;
+    // type 12: a root tree per mode other than the partition's root, so every
+    // mode's MTTKRP is a root traversal (internal; never exported)
+    Csf tree[kMaxDims];
+    bool hasTree[kMaxDims] = {false, false, false, false};
     // type 10
     Hyb hyb;
 };
@@ -182,6 +186,13 @@ struct Factors {
 void mttkrp_csf(const Csf &c, const uint64_t *dims, int outLevel, const Factors &F, float *out,
                 hipStream_t s, hipEvent_t evStart = nullptr, hipEvent_t evEnd = nullptr);
 void mttkrp_coo(const Format &f, const Factors &F, float *out, hipStream_t s);
+// MTTKRP of tensor mode `mode` of a CSF-based format (types 8 and 12),
+// through the mode's root tree when the format has one
+void mttkrp_mode(const Format &f, int mode, const Factors &F, float *out, hipStream_t s,
+                 hipEvent_t evStart = nullptr, hipEvent_t evEnd = nullptr);
+// root-level MTTKRP with one walker per lane (mttkrp_root.hip)
+bool root_walker_supports(int R);
+void mttkrp_root(const Csf &c, const Factors &F, float *out, hipStream_t s);
 void mttkrp_hyb(const Hyb &h, const uint64_t *dims, const Factors &F, float *out, hipStream_t s);
 void fill_zero(float *p, uint64_t n, hipStream_t s);
 

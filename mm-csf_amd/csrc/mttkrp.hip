@@ -519,6 +519,12 @@ void mttkrp_csf(const Csf &c, const uint64_t *dims, int outLevel, const Factors 
     a.nnz = (uint32_t)c.nnz;
     a.R = F.R;
     a.perGroup = per_group_for(c.nnz, g.G);
+    if (outLevel == 0 && root_walker_supports(F.R)) {
+        if (F.dtype == MMCSF_BF16 && F.R % 8 == 0)
+            for (int l = 1; l < N; ++l) check_align(a.U[l], MMCSF_F32, g);  // 16-byte rows
+        mttkrp_root(c, F, out, s);
+        return;
+    }
     if (outLevel > 0) {
         const HotTable &h = c.hot[outLevel];
         MMCSF_REQUIRE(h.tgt.p != nullptr || c.nnz == 0, MMCSF_ERR_ARG,
@@ -543,6 +549,18 @@ void mttkrp_csf(const Csf &c, const uint64_t *dims, int outLevel, const Factors 
             default: launch_csf_t<4, 3>(g, a, F.dtype, s); break;
         }
     }
+}
+
+void mttkrp_mode(const Format &f, int mode, const Factors &F, float *out, hipStream_t s, hipEvent_t evStart,
+                 hipEvent_t evEnd) {
+    if (f.hasTree[mode]) {
+        mttkrp_csf(f.tree[mode], f.dims, 0, F, out, s, evStart, evEnd);
+        return;
+    }
+    int lvl = 0;
+    for (int l = 0; l < f.ndims; ++l)
+        if (f.csf.order[l] == mode) lvl = l;
+    mttkrp_csf(f.csf, f.dims, lvl, F, out, s, evStart, evEnd);
 }
 
 void mttkrp_coo(const Format &f, const Factors &F, float *out, hipStream_t s) {
@@ -645,7 +663,8 @@ void mttkrp_hyb(const Hyb &h, const uint64_t *dims, const Factors &F, float *out
         a.nnz = (uint32_t)c.nnz;
         a.R = F.R;
         a.perGroup = per_group_for(c.nnz, g.G);
-        if (N == 3) launch_csf_t<3, 0>(g, a, F.dtype, s);
+        if (root_walker_supports(F.R)) mttkrp_root(c, F, out, s);
+        else if (N == 3) launch_csf_t<3, 0>(g, a, F.dtype, s);
         else launch_csf_t<4, 0>(g, a, F.dtype, s);
     }
 }

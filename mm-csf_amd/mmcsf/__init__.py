@@ -42,7 +42,7 @@ class Options(C.Structure):
         ("TBsize", C.c_uint64), ("MIfbTh", C.c_uint64), ("fiberPerWarp", C.c_uint64),
         ("verbose", C.c_int), ("correctness", C.c_int), ("nBin", C.c_uint64),
         ("cpdIters", C.c_uint64), ("natOrdering", C.c_int), ("fbrThreashold", C.c_uint64),
-        ("factorDtype", C.c_int), ("seed", C.c_uint64), ("inFileName", C.c_char * 4096), ("outFileName", C.c_char * 4096),
+        ("factorDtype", C.c_int), ("seed", C.c_uint64), ("modeTrees", C.c_int), ("inFileName", C.c_char * 4096), ("outFileName", C.c_char * 4096),
         ("m0", C.c_char * 16), ("m1", C.c_char * 16), ("m2", C.c_char * 16),
     ]
 
@@ -52,7 +52,7 @@ class FormatInfo(C.Structure):
         ("type", C.c_int), ("ndims", C.c_int), ("dims", C.c_uint64 * 4), ("totNnz", C.c_uint64),
         ("modeOrder", C.c_int * 4), ("nLevel", C.c_uint64 * 4), ("nFibers", C.c_uint64),
         ("droppedSlices", C.c_uint64), ("COOnnz", C.c_uint64), ("CSLnnz", C.c_uint64),
-        ("HCSRnnz", C.c_uint64), ("redunMode", C.c_int),
+        ("HCSRnnz", C.c_uint64), ("redunMode", C.c_int), ("modeTrees", C.c_int),
     ]
 
 
@@ -219,9 +219,10 @@ class Tensor:
 class Format:
     """Device format of one driver type (-t 1/3/8/10/12)."""
 
-    def __init__(self, tensor, fmt_type, mode=0, fbr_threshold=99999999, stream=None, opts=None):
+    def __init__(self, tensor, fmt_type, mode=0, fbr_threshold=99999999, stream=None, opts=None, mode_trees=True):
         o = opts if opts is not None else default_options()
         o.mode = int(mode)
+        o.modeTrees = 1 if mode_trees else 0
         o.fbrThreashold = int(fbr_threshold) & 0xFFFFFFFFFFFFFFFF
         h = VP()
         _check(lib().mmcsf_format_build(tensor._h, int(fmt_type), C.byref(o), _stream_ptr(stream),

@@ -57,7 +57,9 @@ def test_cpd_als_matches_float64_als(cuda, R):
     fits, U64, lam64 = D.cpd_als(inds, vals, dims, [u.astype(np.float64) for u in U0], iters, tol=0.0)
     assert st["iterations"] == iters
     np.testing.assert_allclose(st["fit_history"], fits, atol=1e-4, rtol=0)
-    assert st["fit"] > 0.9
+    # the fit counts unobserved entries as zeros, so a sparse sample stays far
+    # from 1; it must still improve over the iterations
+    assert st["fit_history"][-1] > st["fit_history"][0]
     # final model agrees with the float64 model on the observed entries
     mg = D.cp_model_values(inds, [u.cpu().double().numpy() * (lam if m == 0 else 1.0) for m, u in enumerate(Ud)])
     m64 = D.cp_model_values(inds, [u * (lam64 if m == 0 else 1.0) for m, u in enumerate(U64)])

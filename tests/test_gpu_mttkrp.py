@@ -81,12 +81,14 @@ def test_all_types_vs_float64(cuda, dims, nnz, skew, R):
         O.mttkrp_coo_f32([a[perm] for a in inds], vals[perm], m, U, t1)
         tol.append(min(1e-5, max(1e-6, rel(t1, ref[m])) + 1e-6))
     for f in (2, 128):
-        fm = mmcsf.Format(t, 12, fbr_threshold=f)
-        outs, _ = mmcsf.mttkrp_allmode(fm, Ud)
-        for m in range(N):
-            assert rel(outs[m].cpu().numpy(), ref[m]) < tol[m], f"MM-CSF mode {m} f={f}"
-            one = mmcsf.mttkrp(fm, m, Ud).cpu().numpy()
-            assert rel(one, ref[m]) < tol[m]
+        for trees in (True, False):
+            fm = mmcsf.Format(t, 12, fbr_threshold=f, mode_trees=trees)
+            assert fm.info()["modeTrees"] == (N - 1 if trees else 0)
+            outs, _ = mmcsf.mttkrp_allmode(fm, Ud)
+            for m in range(N):
+                assert rel(outs[m].cpu().numpy(), ref[m]) < tol[m], f"MM-CSF mode {m} f={f} trees={trees}"
+                one = mmcsf.mttkrp(fm, m, Ud).cpu().numpy()
+                assert rel(one, ref[m]) < tol[m]
     for m in range(N):
         for ft in (3, 8, 10):
             f = mmcsf.Format(t, ft, mode=m, fbr_threshold=3)
@@ -105,12 +107,13 @@ def test_bf16_factor_storage(cuda, dims, nnz, skew, R):
     Ub = [u.to(torch.bfloat16) for u in Ud]
     Ur = bf16_round(U, torch)
     t = mmcsf.Tensor.from_numpy(dims, inds, vals)
-    fm = mmcsf.Format(t, 12, fbr_threshold=128)
-    outs, _ = mmcsf.mttkrp_allmode(fm, Ub)
-    for m in range(len(dims)):
-        got = outs[m].cpu().numpy()
-        assert rel(got, O.mttkrp_coo_f64(inds, vals, m, U)) < 1e-3
-        assert rel(got, O.mttkrp_coo_f64(inds, vals, m, Ur)) < 1e-5
+    for trees in (True, False):
+        fm = mmcsf.Format(t, 12, fbr_threshold=128, mode_trees=trees)
+        outs, _ = mmcsf.mttkrp_allmode(fm, Ub)
+        for m in range(len(dims)):
+            got = outs[m].cpu().numpy()
+            assert rel(got, O.mttkrp_coo_f64(inds, vals, m, U)) < 1e-3
+            assert rel(got, O.mttkrp_coo_f64(inds, vals, m, Ur)) < 1e-5
     for ft in (3, 8, 10):
         f = mmcsf.Format(t, ft, mode=1, fbr_threshold=128)
         got = mmcsf.mttkrp(f, 1, Ub).cpu().numpy()
@@ -164,11 +167,12 @@ def test_repeat_runs_agree(cuda):
     inds, vals = random_tensor(rng, dims, 30000, [1.2, 1.2, None])
     U, Ud = factors(rng, dims, 32, torch, cuda)
     t = mmcsf.Tensor.from_numpy(dims, inds, vals)
-    fm = mmcsf.Format(t, 12, fbr_threshold=128)
-    a, _ = mmcsf.mttkrp_allmode(fm, Ud)
-    b, _ = mmcsf.mttkrp_allmode(fm, Ud)
-    for m in range(3):
-        assert rel(a[m].cpu().numpy(), b[m].cpu().numpy()) < 1e-6
+    for trees in (True, False):
+        fm = mmcsf.Format(t, 12, fbr_threshold=128, mode_trees=trees)
+        a, _ = mmcsf.mttkrp_allmode(fm, Ud)
+        b, _ = mmcsf.mttkrp_allmode(fm, Ud)
+        for m in range(3):
+            assert rel(a[m].cpu().numpy(), b[m].cpu().numpy()) < 1e-6
 
 
 def test_error_paths(cuda):
