@@ -263,6 +263,40 @@ int mmcsf_cpd_als(const mmcsf_format *f, int R, int max_iters, double tol,
  * (the ssyrk_ calls of cpd_cpu.cu:543-546, 649-652). */
 int mmcsf_gram(const float *U, uint64_t rows, int R, double *G, void *stream);
 
+/* ---- CP completion (no reference counterpart) ---------------------------------
+ *
+ * Squared loss of a CP model on the observed entries and its gradients, for
+ * tensor completion with a gradient optimiser:
+ *     e_x = sum_r prod_n U_n[i_n(x)][r] - v_x,    L = 1/2 sum_x e_x^2,
+ *     dL/dU_n = MTTKRP_n(e)  (mttkrp_cpu.cpp:60-113 with e in place of v).
+ * Each call is one lane-walker pass over a CSF tree of the format, computing
+ * e_x on the fly (float32 arithmetic, float64 loss sum).  The format must be
+ * B-CSF (its root mode only) or MM-CSF (every mode when built with mode
+ * trees, MMCSF_TYPE_MMCSF with modeTrees = 1).  R % 4 == 0 and R <= 128;
+ * factor rows 16-byte aligned (8-byte for bfloat16 with R % 8 != 0).
+ * Residuals and weights are in the order of the tensor the format was built
+ * from.  `loss` is a DEVICE float64 scalar, written stream-ordered. */
+
+/* loss = 1/2 sum e^2; residual (device float[nnz], may be NULL) = e. */
+int mmcsf_cp_forward(const mmcsf_format *f, int R, int dtype,
+                     const void *const *factors, float *residual,
+                     double *loss, void *stream);
+
+/* grad (device float dims[mode] x R, overwritten) =
+ *     alpha * sum_x w_x * prod_{m != mode} U_m[i_m(x)]  (row i_mode(x)),
+ * with w = weights (device float[nnz]) -- the vector-Jacobian product of
+ * the model values -- or, when weights is NULL, w = e recomputed from the
+ * factors, so that grad = alpha * dL/dU_mode. */
+int mmcsf_cp_adjoint(const mmcsf_format *f, int mode, int R, int dtype,
+                     const void *const *factors, const float *weights,
+                     float alpha, float *grad, void *stream);
+
+/* loss = 1/2 sum e^2 and grads[m] = alpha * dL/dU_m for every m with
+ * grads[m] != NULL, one pass per mode (the loss is summed in the first). */
+int mmcsf_cp_loss_grad(const mmcsf_format *f, int R, int dtype,
+                       const void *const *factors, float alpha,
+                       float *const *grads, double *loss, void *stream);
+
 /* ---- version ---------------------------------------------------------------- */
 const char *mmcsf_version(void);
 

@@ -377,9 +377,11 @@ int mmcsf_format_build(const mmcsf_tensor *th, int type, const mmcsf_options *op
             // every nonzero lands in partition 2 (util.h:2517), whose mode
             // order is order_tensormode(X, 2) (init_tensor, util.h:372-387)
             order_tensormode(t.ndims, t.dims, 2, nat, order);
-            sort_coo(t, order, keys, vals, s);
+            DevArray<uint32_t> pos;
+            sort_coo(t, order, keys, vals, s, &pos);
             uint64_t fs = t.ndims == 4 ? 128 : kNoSplit;  // util.h:1325-1326
             build_csf(t.ndims, t.dims, t.nnz, order, keys, std::move(vals), opt->fbrThreashold, fs, f.csf, s);
+            f.csf.pos = std::move(pos);
             if (opt->modeTrees) {
                 // a root tree for every other mode: each MTTKRP becomes a root traversal
                 for (int m = 0; m < t.ndims; ++m) {
@@ -387,10 +389,12 @@ int mmcsf_format_build(const mmcsf_tensor *th, int type, const mmcsf_options *op
                     int o2[kMaxDims];
                     DevArray<uint32_t> k2[kMaxDims];
                     DevArray<float> v2;
+                    DevArray<uint32_t> p2;
                     order_tensormode(t.ndims, t.dims, m, nat, o2);
-                    sort_coo(t, o2, k2, v2, s);
+                    sort_coo(t, o2, k2, v2, s, &p2);
                     build_csf(t.ndims, t.dims, t.nnz, o2, k2, std::move(v2), opt->fbrThreashold, kNoSplit, f.tree[m],
                               s);
+                    f.tree[m].pos = std::move(p2);
                     f.hasTree[m] = true;
                 }
             } else {
@@ -403,10 +407,12 @@ int mmcsf_format_build(const mmcsf_tensor *th, int type, const mmcsf_options *op
                 for (int l = 0; l < t.ndims; ++l) f.cooOrder[l] = order[l];
                 sort_coo(t, order, f.cooKeys, f.cooVals, s);
             } else {
-                sort_coo(t, order, keys, vals, s);
+                DevArray<uint32_t> pos;
+                sort_coo(t, order, keys, vals, s, type == MMCSF_TYPE_BCSF ? &pos : nullptr);
                 build_csf(t.ndims, t.dims, t.nnz, order, keys, std::move(vals), opt->fbrThreashold, kNoSplit,
                           f.csf, s);
                 if (type == MMCSF_TYPE_BCSF) {
+                    f.csf.pos = std::move(pos);
                     build_bins_tiled(f.csf, s);
                 } else {
                     build_hyb(t.ndims, t.dims, std::move(f.csf), f.hyb, s);
@@ -666,6 +672,95 @@ int mmcsf_gram(const float *U, uint64_t rows, int R, double *G, void *stream) {
         MMCSF_REQUIRE(U && G, MMCSF_ERR_ARG, "null argument");
         MMCSF_REQUIRE(R >= 1 && R <= 256 && rows <= kMaxIndex, MMCSF_ERR_UNSUPPORTED, "unsupported shape");
         gram(U, rows, R, G, static_cast<hipStream_t>(stream));
+    });
+}
+
+// ---- CP completion -----------------------------------------------------------
+
+namespace {
+
+// the tree whose root level is `mode` (with tensor-order positions), or null
+const Csf *root_tree(const Format &f, int mode) {
+    if (f.type == MMCSF_TYPE_MMCSF) {
+        if (f.csf.order[0] == mode) return &f.csf;
+        if (f.hasTree[mode]) return &f.tree[mode];
+        return nullptr;
+    }
+    if (f.type == MMCSF_TYPE_BCSF && f.csf.order[0] == mode) return &f.csf;
+    return nullptr;
+}
+
+Factors cp_factors(const Format &f, int R, int dtype, const void *const *factors) {
+    MMCSF_REQUIRE(factors, MMCSF_ERR_ARG, "null argument");
+    MMCSF_REQUIRE(dtype == MMCSF_F32 || dtype == MMCSF_BF16, MMCSF_ERR_ARG, "dtype must be F32 or BF16");
+    MMCSF_REQUIRE(cp_supports(R), MMCSF_ERR_UNSUPPORTED, "CP completion needs R % 4 == 0 and R <= 128");
+    const size_t need = dtype == MMCSF_F32 || R % 8 == 0 ? 16 : 8;
+    Factors F{};
+    for (int m = 0; m < f.ndims; ++m) {
+        MMCSF_REQUIRE(factors[m], MMCSF_ERR_ARG, "missing factor matrix");
+        MMCSF_REQUIRE(reinterpret_cast<uintptr_t>(factors[m]) % need == 0, MMCSF_ERR_ARG,
+                      "factor pointer is not aligned to " + std::to_string(need) + " bytes");
+        F.U[m] = factors[m];
+    }
+    F.dtype = dtype;
+    F.R = R;
+    return F;
+}
+
+}  // namespace
+
+int mmcsf_cp_forward(const mmcsf_format *fh, int R, int dtype, const void *const *factors, float *residual,
+                     double *loss, void *stream) {
+    return guarded([&] {
+        MMCSF_REQUIRE(fh && loss, MMCSF_ERR_ARG, "null argument");
+        const Format &f = fh->f;
+        MMCSF_REQUIRE(f.type == MMCSF_TYPE_MMCSF || f.type == MMCSF_TYPE_BCSF, MMCSF_ERR_ARG,
+                      "CP completion needs a B-CSF or MM-CSF format");
+        const Factors F = cp_factors(f, R, dtype, factors);
+        MMCSF_REQUIRE(f.csf.pos.p || f.nnz == 0, MMCSF_ERR_ARG, "format holds no nonzero positions");
+        cp_forward(f.csf, F, residual, loss, static_cast<hipStream_t>(stream));
+    });
+}
+
+int mmcsf_cp_adjoint(const mmcsf_format *fh, int mode, int R, int dtype, const void *const *factors,
+                     const float *weights, float alpha, float *grad, void *stream) {
+    return guarded([&] {
+        MMCSF_REQUIRE(fh && grad, MMCSF_ERR_ARG, "null argument");
+        const Format &f = fh->f;
+        MMCSF_REQUIRE(mode >= 0 && mode < f.ndims, MMCSF_ERR_ARG, "Mode cannot be larger than tensor dimension.");
+        const Factors F = cp_factors(f, R, dtype, factors);
+        const Csf *c = root_tree(f, mode);
+        MMCSF_REQUIRE(c, MMCSF_ERR_ARG, "the format holds no tree rooted at this mode");
+        MMCSF_REQUIRE(c->pos.p || f.nnz == 0, MMCSF_ERR_ARG, "format holds no nonzero positions");
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        fill_zero(grad, f.dims[mode] * (uint64_t)R, s);
+        cp_gradient(*c, F, weights, alpha, grad, nullptr, s);
+    });
+}
+
+int mmcsf_cp_loss_grad(const mmcsf_format *fh, int R, int dtype, const void *const *factors, float alpha,
+                       float *const *grads, double *loss, void *stream) {
+    return guarded([&] {
+        MMCSF_REQUIRE(fh && grads && loss, MMCSF_ERR_ARG, "null argument");
+        const Format &f = fh->f;
+        const Factors F = cp_factors(f, R, dtype, factors);
+        for (int m = 0; m < f.ndims; ++m)
+            MMCSF_REQUIRE(!grads[m] || root_tree(f, m), MMCSF_ERR_ARG,
+                          "all-mode gradients need an MM-CSF format with a root tree per mode");
+        MMCSF_REQUIRE(f.type == MMCSF_TYPE_MMCSF || f.type == MMCSF_TYPE_BCSF, MMCSF_ERR_ARG,
+                      "CP completion needs a B-CSF or MM-CSF format");
+        MMCSF_REQUIRE(f.csf.pos.p || f.nnz == 0, MMCSF_ERR_ARG, "format holds no nonzero positions");
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        cp_zero(loss, s);
+        bool counted = false;
+        for (int m = 0; m < f.ndims; ++m) {
+            if (!grads[m]) continue;
+            fill_zero(grads[m], f.dims[m] * (uint64_t)R, s);
+            cp_gradient(*root_tree(f, m), F, nullptr, alpha, grads[m], counted ? nullptr : loss, s);
+            counted = true;
+        }
+        if (counted) cp_half(loss, s);
+        else cp_forward(f.csf, F, nullptr, loss, s);
     });
 }
 

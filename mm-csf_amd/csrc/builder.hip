@@ -342,12 +342,13 @@ void order_tensormode(int ndims, const uint64_t *dims, int mode, bool nat, int *
 // full-key LSD radix sort of the nonzeros
 // ---------------------------------------------------------------------------
 void sort_coo(const Tensor &t, const int *order, DevArray<uint32_t> *keys, DevArray<float> &vals,
-              hipStream_t s) {
+              hipStream_t s, DevArray<uint32_t> *pos) {
     const int N = t.ndims;
     const uint64_t n = t.nnz;
     if (n == 0) {
         for (int l = 0; l < N; ++l) keys[l].alloc(0);
         vals.alloc(0);
+        if (pos) pos->alloc(0);
         return;
     }
     Temp tmp;
@@ -393,6 +394,7 @@ void sort_coo(const Tensor &t, const int *order, DevArray<uint32_t> *keys, DevAr
     vals.alloc(n);
     hipLaunchKernelGGL(k_gather_f32, dim3(nblk(n)), dim3(kBS), 0, s, t.vals.p, perm.p, vals.p, n);
     MMCSF_HIP(hipStreamSynchronize(s));
+    if (pos) *pos = std::move(perm);
 }
 
 // ---------------------------------------------------------------------------

@@ -108,6 +108,8 @@ struct Csf {
     DevArray<uint32_t> idx[kMaxDims - 1];
     DevArray<uint32_t> leaf;             // coordinate of order[N-1] per nonzero
     DevArray<float> vals;
+    DevArray<uint32_t> pos;              // tensor-order position of every stored nonzero
+                                         // (B-CSF and MM-CSF trees; empty otherwise)
     DevArray<uint32_t> fbrSlc;           // fbrLikeSlcInds: slice coord per level-1 entry
     // slice bins (make_TiledBin / make_HybBin)
     bool hasBins = false;
@@ -160,8 +162,9 @@ void order_tensormode(int ndims, const uint64_t *dims, int mode, bool nat, int *
 
 // Sorts the tensor's nonzeros by the full key in `order`; outputs level keys
 // keys[l] = coordinate of order[l], and permuted values.
+// pos (optional) receives the tensor-order position of every sorted nonzero.
 void sort_coo(const Tensor &t, const int *order, DevArray<uint32_t> *keys, DevArray<float> &vals,
-              hipStream_t s);
+              hipStream_t s, DevArray<uint32_t> *pos = nullptr);
 
 // CSF scan with the -f split and (4-D) fbrS split; consumes sorted keys.
 void build_csf(int ndims, const uint64_t *dims, uint64_t nnz, const int *order,
@@ -195,6 +198,15 @@ bool root_walker_supports(int R);
 void mttkrp_root(const Csf &c, const Factors &F, float *out, hipStream_t s);
 void mttkrp_hyb(const Hyb &h, const uint64_t *dims, const Factors &F, float *out, hipStream_t s);
 void fill_zero(float *p, uint64_t n, hipStream_t s);
+
+// CP completion (cp.hip): loss 1/2 sum e^2 with e = model - value, residuals
+// in tensor order, gradients of the root mode of a tree
+bool cp_supports(int R);
+void cp_forward(const Csf &c, const Factors &F, float *resid, double *loss, hipStream_t s);
+void cp_gradient(const Csf &c, const Factors &F, const float *weights, float alpha, float *grad, double *loss,
+                 hipStream_t s);
+void cp_half(double *loss, hipStream_t s);
+void cp_zero(double *loss, hipStream_t s);
 
 // CP-ALS (cpd.hip)
 void gram(const float *U, uint64_t rows, int R, double *G, hipStream_t s);

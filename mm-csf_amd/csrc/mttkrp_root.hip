@@ -18,7 +18,7 @@
 #include <cstdlib>
 #include <type_traits>
 
-#include "mmcsf_internal.hpp"
+#include "root_walk.hpp"
 
 namespace mmcsf {
 
@@ -36,82 +36,14 @@ struct RootArgs {
 };
 
 namespace {
-
-__device__ __forceinline__ uint32_t upper_idx(const uint32_t *__restrict__ a, uint32_t n, uint32_t key) {
-    uint32_t lo = 0, hi = n;
-    while (hi - lo > 1) {
-        uint32_t mid = lo + ((hi - lo) >> 1);
-        if (a[mid] <= key) lo = mid;
-        else hi = mid;
-    }
-    return lo;
-}
-
-template <int CPL, typename T>
-struct LaneRow {
-    float v[CPL];
-    __device__ __forceinline__ void zero() {
-#pragma unroll
-        for (int c = 0; c < CPL; ++c) v[c] = 0.f;
-    }
-    // columns [c0, c0 + CPL) of row `row`; R % 4 == 0 and c0 % 4 == 0
-    __device__ __forceinline__ void load(const T *__restrict__ U, uint32_t row, int R, int c0) {
-        const T *p = U + (size_t)row * R + c0;
-        if constexpr (std::is_same<T, float>::value) {
-#pragma unroll
-            for (int c = 0; c < CPL; c += 4) {
-                if (c0 + c < R) {
-                    float4 q = *reinterpret_cast<const float4 *>(p + c);
-                    v[c] = q.x; v[c + 1] = q.y; v[c + 2] = q.z; v[c + 3] = q.w;
-                } else {
-                    v[c] = v[c + 1] = v[c + 2] = v[c + 3] = 0.f;
-                }
-            }
-        } else {
-            const bool wide = (R & 7) == 0;  // 16-byte aligned rows
-#pragma unroll
-            for (int c = 0; c < CPL; c += 8) {
-                if (c0 + c < R) {
-                    if (wide) {
-                        uint4 q = *reinterpret_cast<const uint4 *>(p + c);
-                        v[c] = __uint_as_float(q.x << 16); v[c + 1] = __uint_as_float(q.x & 0xffff0000u);
-                        v[c + 2] = __uint_as_float(q.y << 16); v[c + 3] = __uint_as_float(q.y & 0xffff0000u);
-                        v[c + 4] = __uint_as_float(q.z << 16); v[c + 5] = __uint_as_float(q.z & 0xffff0000u);
-                        v[c + 6] = __uint_as_float(q.w << 16); v[c + 7] = __uint_as_float(q.w & 0xffff0000u);
-                    } else {  // 8-byte aligned rows: two 4-column loads
-                        uint2 q = *reinterpret_cast<const uint2 *>(p + c);
-                        v[c] = __uint_as_float(q.x << 16); v[c + 1] = __uint_as_float(q.x & 0xffff0000u);
-                        v[c + 2] = __uint_as_float(q.y << 16); v[c + 3] = __uint_as_float(q.y & 0xffff0000u);
-                        if (c0 + c + 4 < R) {
-                            uint2 q2 = *reinterpret_cast<const uint2 *>(p + c + 4);
-                            v[c + 4] = __uint_as_float(q2.x << 16); v[c + 5] = __uint_as_float(q2.x & 0xffff0000u);
-                            v[c + 6] = __uint_as_float(q2.y << 16); v[c + 7] = __uint_as_float(q2.y & 0xffff0000u);
-                        } else {
-                            v[c + 4] = v[c + 5] = v[c + 6] = v[c + 7] = 0.f;
-                        }
-                    }
-                } else {
-#pragma unroll
-                    for (int b = 0; b < 8; ++b) v[c + b] = 0.f;
-                }
-            }
-        }
-    }
-};
-
-template <int CPL>
-__device__ __forceinline__ void atomic_row(float *__restrict__ out, uint32_t row, int R, int c0, const float *v) {
-    float *p = out + (size_t)row * R + c0;
-#pragma unroll
-    for (int c = 0; c < CPL; ++c)
-        if (c0 + c < R) unsafeAtomicAdd(p + c, v[c]);
-}
+using namespace walk;
 
 template <int N, int CPL, int G, typename T>
 __global__ __launch_bounds__(256) void k_root(RootArgs a) {
     using Row = LaneRow<CPL, T>;
     constexpr int LF = N - 2;
     const int R = a.R;
+    static_assert(G > 0 && (G & (G - 1)) == 0 && G <= 64, "walker groups stay inside one wave");
     const int sub = threadIdx.x % G;
     const int c0 = sub * CPL;
     const uint64_t walkers = (uint64_t)gridDim.x * blockDim.x / G;
@@ -207,6 +139,13 @@ __global__ __launch_bounds__(256) void k_root(RootArgs a) {
             for (int c = 0; c < CPL; ++c) aS[c] = __builtin_fmaf(aJ.v[c], W.v[c], aS[c]);
         }
     }
+    else if (x0 < a.nnz) {
+        // a walker without nonzeros takes the slice at its position, so the
+        // slice keys stay non-decreasing across the wave for the reduction
+        uint32_t f = upper_idx(ptrL, a.cnt[LF], x0);
+        if constexpr (N == 4) f = upper_idx(a.ptr[1], a.cnt[1], f);
+        slc = a.slc[f];
+    }
     // segmented reduction of the last slice across the walkers of the wave
     const int lane = threadIdx.x & 63;
 #pragma unroll
@@ -237,7 +176,9 @@ Cfg pick(int R) {
     if (R <= 8) return {8, 1};
     if (R <= 16) return {16, 1};
     if (R <= 32) return {32, 1};
-    return {32, (R + 31) / 32};
+    if (R <= 64) return {32, 2};
+    if (R <= 128) return {32, 4};
+    return {32, 8};
 }
 
 template <int N, int CPL, int G, typename T>
@@ -270,11 +211,7 @@ void dispatch(const Cfg &c, const RootArgs &a, hipStream_t s) {
     MMCSF_ROOT(16, 4)
     MMCSF_ROOT(32, 1)
     MMCSF_ROOT(32, 2)
-    MMCSF_ROOT(32, 3)
     MMCSF_ROOT(32, 4)
-    MMCSF_ROOT(32, 5)
-    MMCSF_ROOT(32, 6)
-    MMCSF_ROOT(32, 7)
     MMCSF_ROOT(32, 8)
 #undef MMCSF_ROOT
     fail(MMCSF_ERR_UNSUPPORTED, "no root kernel for this rank");

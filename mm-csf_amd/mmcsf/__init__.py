@@ -106,6 +106,9 @@ def lib():
     L.mmcsf_gram.argtypes = [VP, C.c_uint64, C.c_int, VP, VP]
     L.mmcsf_mttkrp_coo_host.argtypes = [VP, C.c_int, C.c_int, C.POINTER(C.POINTER(C.c_float)),
                                         C.POINTER(C.c_float)]
+    L.mmcsf_cp_forward.argtypes = [VP, C.c_int, C.c_int, C.POINTER(VP), VP, VP, VP]
+    L.mmcsf_cp_adjoint.argtypes = [VP, C.c_int, C.c_int, C.c_int, C.POINTER(VP), VP, C.c_float, VP, VP]
+    L.mmcsf_cp_loss_grad.argtypes = [VP, C.c_int, C.c_int, C.POINTER(VP), C.c_float, C.POINTER(VP), VP, VP]
     _LIB = L
     return L
 
@@ -238,6 +241,7 @@ class Format:
             "modeOrder": list(i.modeOrder[: i.ndims]), "nLevel": list(i.nLevel[: i.ndims]),
             "nFibers": i.nFibers, "droppedSlices": i.droppedSlices, "COOnnz": i.COOnnz,
             "CSLnnz": i.CSLnnz, "HCSRnnz": i.HCSRnnz, "redunMode": i.redunMode,
+            "modeTrees": i.modeTrees,
         }
 
     def export(self, name, k=0):
@@ -357,3 +361,107 @@ def cpd_als(fmt, factors, iters=10, tol=1e-5, mttkrp_bf16=False, stream=None):
         "iterations": n, "fit": st.fit, "fit_history": list(st.fit_history[: min(n, 64)]),
         "ms_total": st.ms_total, "ms_mttkrp": st.ms_mttkrp, "lu_fallbacks": st.lu_fallbacks,
     }
+
+
+# ---- CP completion -------------------------------------------------------------
+
+def _cp_factors(fmt, factors):
+    info = fmt.info()
+    R = factors[0].shape[1]
+    _check_factors(info["dims"], factors, R)
+    return info, R, _dtype_of(factors)
+
+
+def cp_forward(fmt, factors, residual=False, stream=None):
+    """Loss 1/2 sum_x e_x^2 of the CP model on the observed entries, with
+    e_x = sum_r prod_n U_n[i_n(x)][r] - v_x.  Returns (loss, residual): loss is
+    a device float64 scalar tensor, residual (float32, tensor order) or None."""
+    import torch
+    info, R, dt = _cp_factors(fmt, factors)
+    dev = factors[0].device
+    loss = torch.empty((), dtype=torch.float64, device=dev)
+    res = torch.empty(info["totNnz"], dtype=torch.float32, device=dev) if residual else None
+    _check(lib().mmcsf_cp_forward(fmt._h, int(R), dt, _factor_ptrs(factors),
+                                  VP(res.data_ptr()) if res is not None else VP(0), VP(loss.data_ptr()),
+                                  _stream_ptr(stream)))
+    return loss, res
+
+
+def cp_adjoint(fmt, mode, factors, weights=None, alpha=1.0, out=None, stream=None):
+    """alpha * sum_x w_x * d(model_x)/dU_mode as a (dims[mode], R) float32 tensor.
+    ``weights`` (float32, tensor order) or None for w = the residual, which
+    makes the result alpha * dL/dU_mode."""
+    import torch
+    info, R, dt = _cp_factors(fmt, factors)
+    rows = info["dims"][mode]
+    if out is None:
+        out = torch.empty((rows, R), dtype=torch.float32, device=factors[0].device)
+    if weights is not None:
+        if (not weights.is_cuda or weights.dtype != torch.float32 or not weights.is_contiguous()
+                or weights.numel() != info["totNnz"]):
+            raise ValueError("weights must be a contiguous float32 device tensor with one entry per nonzero")
+    _check(lib().mmcsf_cp_adjoint(fmt._h, int(mode), int(R), dt, _factor_ptrs(factors),
+                                  VP(weights.data_ptr()) if weights is not None else VP(0), float(alpha),
+                                  VP(out.data_ptr()), _stream_ptr(stream)))
+    return out
+
+
+def cp_loss_grad(fmt, factors, alpha=1.0, need=None, stream=None):
+    """(loss, grads): loss as in cp_forward, grads[m] = alpha * dL/dU_m (float32)
+    for every m with need[m] (default: all), else None; one pass per mode."""
+    import torch
+    info, R, dt = _cp_factors(fmt, factors)
+    dev = factors[0].device
+    need = need if need is not None else [True] * len(factors)
+    grads = [torch.empty((d, R), dtype=torch.float32, device=dev) if n else None
+             for d, n in zip(info["dims"], need)]
+    loss = torch.empty((), dtype=torch.float64, device=dev)
+    gp = (VP * len(grads))(*[VP(g.data_ptr()) if g is not None else VP(0) for g in grads])
+    _check(lib().mmcsf_cp_loss_grad(fmt._h, int(R), dt, _factor_ptrs(factors), float(alpha), gp,
+                                    VP(loss.data_ptr()), _stream_ptr(stream)))
+    return loss, grads
+
+
+def _autograd_function():
+    import torch
+
+    class CPCompletionLoss(torch.autograd.Function):
+        """L(U_0..U_{N-1}) = 1/2 sum_x (sum_r prod_n U_n[i_n(x)][r] - v_x)^2 on the
+        observed entries of an MM-CSF format.  The forward pass computes the loss
+        and, when a gradient is needed, every factor's gradient in the same
+        per-mode traversals; backward scales them by the incoming gradient."""
+
+        @staticmethod
+        def forward(ctx, fmt, *factors):
+            need = list(ctx.needs_input_grad[1:])
+            if any(need):
+                loss, grads = cp_loss_grad(fmt, list(factors), need=need)
+                ctx.save_for_backward(*[g if g is not None else torch.empty(0) for g in grads])
+                ctx.need = need
+                ctx.dtypes = [f.dtype for f in factors]
+            else:
+                loss, _ = cp_forward(fmt, list(factors))
+            return loss
+
+        @staticmethod
+        def backward(ctx, gout):
+            grads = ctx.saved_tensors
+            out = [None]
+            for u_need, g, dt in zip(ctx.need, grads, ctx.dtypes):
+                out.append((g * gout.to(g.dtype)).to(dt) if u_need else None)
+            return tuple(out)
+
+    return CPCompletionLoss
+
+
+_CP_FN = None
+
+
+def cp_completion_loss(fmt, factors):
+    """Differentiable CP-completion loss (float64 device scalar).  Factors are
+    device tensors (all float32 or all bfloat16); their gradients come back
+    in float32 for float32 factors and in bfloat16 for bfloat16 factors."""
+    global _CP_FN
+    if _CP_FN is None:
+        _CP_FN = _autograd_function()
+    return _CP_FN.apply(fmt, *factors)
