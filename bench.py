@@ -38,17 +38,18 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def algorithmic_bytes(info, distinct, R, sf):
+def algorithmic_bytes(info, distinct, R, sf, single_tree=False):
     """Compulsory HBM bytes of each mode's MTTKRP (4-byte indices/values):
-    nonzeros (value + leaf index), fiber levels (pointer + index), the slice
-    coordinate of each level-1 entry, each distinct input factor row read
-    once (sf bytes per element) and each distinct output row written once."""
+    nonzeros (value + leaf index), fiber levels (pointer + index) and the
+    slice coordinate of each level-1 entry of the tree that mode traverses,
+    each distinct input factor row read once (sf bytes per element) and each
+    distinct output row written once."""
     N = info["ndims"]
     nnz = info["totNnz"]
-    lv = info["nLevel"]
-    struct = nnz * 8 + sum(lv[l] * 8 for l in range(1, N - 1)) + lv[1] * 4
     out = []
     for n in range(N):
+        lv = info["nLevel"] if single_tree else info["modeLevel"][n]
+        struct = nnz * 8 + sum(lv[l] * 8 for l in range(1, N - 1)) + lv[1] * 4
         fac = sum(distinct[m] * R * sf for m in range(N) if m != n)
         out.append(struct + fac + distinct[n] * R * 4)
     return out
@@ -125,26 +126,12 @@ def cpu_baseline(inds, vals, U, dims, sample, bytes_f32):
     }
 
 
-def main():
-    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--nnz", type=int, default=100_000_000)
-    ap.add_argument("--cpu-sample", type=int, default=50_000_000)
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print("bench.py measures one GPU: every GPU box here has one GPU and the multi-GPU "
-              "path is not built (see DESIGN.md, sharding)", file=sys.stderr)
-        sys.exit(2)
-
+def run_c2(args):
     import torch
 
     import mmcsf
 
     dev = torch.device("cuda:0")
-    torch.cuda.set_device(dev)
     dims = [500000, 200000, 50000]
     R = 32
     log("generating tensor")
@@ -207,6 +194,7 @@ def main():
         _, ms = mmcsf.mttkrp_allmode(fmt1, U, outs, timed=True)
         single_steps.append(ms)
     single_k = [sum(s[m] for s in single_steps) / len(single_steps) for m in range(3)]
+    bytes_single = algorithmic_bytes(fmt1.info(), distinct, R, 4, single_tree=True)
     del fmt1
     torch.cuda.empty_cache()
     log(f"single tree: kernels {single_k}")
@@ -264,7 +252,7 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": f"mode {dom} (level {info['modeOrder'].index(dom)}) MM-CSF MTTKRP, float32 factors",
+            "kernel": f"mode {dom} MTTKRP (root lane-walker k_root over the mode's root tree), float32 factors",
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
@@ -275,7 +263,7 @@ def main():
             "note": "the reference's MM-CSF layout: one tree, modes other than its root scatter",
             "kernel_ms": [round(k, 4) for k in single_k],
             "all_mode_kernel_ms": round(sum(single_k), 4),
-            "gbs": round(sum(bytes_f32) / (sum(single_k) * 1e-3) / 1e9, 2),
+            "gbs": round(sum(bytes_single) / (sum(single_k) * 1e-3) / 1e9, 2),
         },
         "generate_s": round(gen_s, 2),
         "build_s": round(build_s, 2),
@@ -284,6 +272,326 @@ def main():
         log("cpu baseline")
         result["cpu_baseline"] = cpu_baseline(inds, vals, U, dims, args.cpu_sample, bytes_f32)
     print(json.dumps(result), flush=True)
+
+
+# ---------------------------------------------------------------------------
+# further workloads (parity cases of the path, each printing its own line)
+# ---------------------------------------------------------------------------
+def _event_ms(torch, fn, steps, warmup):
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps):
+        fn()
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / steps
+
+
+def _line(metric, value, unit, ms, args, dtype, config, higher=True, **extra):
+    d = {"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": args.steps,
+         "warmup": args.warmup, "ms_per_step": round(ms, 4), "higher_is_better": higher, "scaling": "weak",
+         "vs_baseline": None, "dtype": dtype, "data": "synthetic (GPU generator)", "config": config}
+    d.update(extra)
+    print(json.dumps(d), flush=True)
+
+
+def run_c1(args):
+    """Mode-0 MTTKRP on a uniform 2000^3 tensor with 1M nonzeros, R=32, -f 128:
+    COO (-t 3), B-CSF (-t 8), HB-CSF (-t 10), MM-CSF (-t 12) against the
+    float64 contraction and the sequential float32 COO CPU MTTKRP (-t 1)."""
+    import numpy as np
+    import torch
+
+    import mmcsf
+    import oracle as O  # the -t 1 CPU leg
+
+    dev = torch.device("cuda:0")
+    dims, nnz, R = [2000, 2000, 2000], 1_000_000, 32
+    X = mmcsf.Tensor.generate(dims, nnz, [0, 0, 0], [0.0, 0.0, 0.0], 0x4D4D43534600)
+    inds, vals = X.to_torch(dev)
+    g = torch.Generator(device=dev).manual_seed(0x4D4D43534600)
+    U = [torch.rand((d, R), generator=g, device=dev) for d in dims]
+    ref = fp64_reference(inds, vals, U, 0, dims)
+    hi = [a.cpu().numpy().astype(np.int64) for a in inds]
+    hv = vals.cpu().numpy()
+    order = O.order_tensormode(dims, 0)
+    perm = np.lexsort((hi[order[1]], hi[order[0]]))
+    t1 = np.zeros((dims[0], R), dtype=np.float32)
+    c0 = time.perf_counter()
+    O.mttkrp_coo_f32([a[perm] for a in hi], hv[perm], 0, [u.cpu().numpy() for u in U], t1)
+    t1_s = time.perf_counter() - c0
+    t1d = torch.from_numpy(t1).to(dev).double()
+    t1_err = rel_err(t1d.float(), ref)[0]
+    types = {}
+    out = torch.empty((dims[0], R), device=dev)
+    for typ, name in ((3, "COO"), (8, "B-CSF"), (10, "HB-CSF"), (12, "MM-CSF")):
+        fmt = mmcsf.Format(X, typ, mode=0, fbr_threshold=128)
+        ms = _event_ms(torch, lambda: mmcsf.mttkrp(fmt, 0, U, out), args.steps, args.warmup)
+        e64 = rel_err(out, ref)[0]
+        et1 = float(torch.linalg.norm(out.double() - t1d) / torch.linalg.norm(t1d))
+        types[name] = {"type": typ, "ms": round(ms, 4), "gflops": round(3 * nnz * R / (ms * 1e-3) / 1e9, 1),
+                       "rel_err_fp64": e64, "rel_err_vs_t1": et1}
+        log(f"c1 {name}: {ms:.4f} ms, err fp64 {e64:.2e}, vs -t 1 {et1:.2e}")
+        del fmt
+    best = min(types.values(), key=lambda v: v["ms"])
+    _line("Mode-0 MTTKRP ms, uniform 2000^3, 1M nonzeros, R=32", best["ms"], "ms", best["ms"], args, "f32",
+          {"workload": "mode-0 MTTKRP, 2000x2000x2000 uniform, 1000000 distinct nonzeros, R=32, -f 128",
+           "R": R, "nnz": nnz, "dims": dims}, higher=False, types=types,
+          t1_cpu={"ms": round(t1_s * 1e3, 2), "cores": 1, "rel_err_fp64": t1_err})
+
+
+def run_c3(args):
+    """4-mode precision sweep: 100000x50000x20000x1000, 50M nonzeros, Zipf1.1 on
+    modes 0 and 2; all-mode MTTKRP at R=16 and R=64, float32 vs bfloat16
+    factor storage, errors against float64."""
+    import torch
+
+    import mmcsf
+
+    dev = torch.device("cuda:0")
+    dims, nnz = [100000, 50000, 20000, 1000], args.nnz_c3
+    X = mmcsf.Tensor.generate(dims, nnz, [1, 0, 1, 0], [1.1, 0.0, 1.1, 0.0], 0x4D4D43534602)
+    t0 = time.perf_counter()
+    fmt = mmcsf.Format(X, mmcsf.TYPE_MMCSF, fbr_threshold=128)
+    torch.cuda.synchronize()
+    build_s = time.perf_counter() - t0
+    info = fmt.info()
+    inds, vals = X.to_torch(dev)
+    distinct = [int((torch.bincount(inds[m].long(), minlength=dims[m]) > 0).sum()) for m in range(4)]
+    runs = []
+    for R in (16, 64):
+        g = torch.Generator(device=dev).manual_seed(0x4D4D43534602 + R)
+        U = [torch.rand((d, R), generator=g, device=dev) for d in dims]
+        Ub = [u.to(torch.bfloat16) for u in U]
+        outs = [torch.empty((d, R), device=dev) for d in dims]
+        f32 = _event_ms(torch, lambda: mmcsf.mttkrp_allmode(fmt, U, outs), args.steps, args.warmup)
+        o32 = [o.clone() for o in outs]
+        _, k32 = mmcsf.mttkrp_allmode(fmt, U, outs, timed=True)
+        b16 = _event_ms(torch, lambda: mmcsf.mttkrp_allmode(fmt, Ub, outs), args.steps, args.warmup)
+        o16 = [o.clone() for o in outs]
+        _, k16 = mmcsf.mttkrp_allmode(fmt, Ub, outs, timed=True)
+        e32, e16 = [], []
+        for m in range(4):
+            ref = fp64_reference(inds, vals, U, m, dims, chunk=2_000_000)
+            e32.append(rel_err(o32[m], ref)[0])
+            e16.append(rel_err(o16[m], ref)[0])
+            del ref
+        torch.cuda.empty_cache()
+        b32 = algorithmic_bytes(info, distinct, R, 4)
+        bb = algorithmic_bytes(info, distinct, R, 2)
+        runs.append({"R": R, "f32_ms": round(f32, 4), "bf16_ms": round(b16, 4),
+                     "f32_gbs": round(sum(b32) / (f32 * 1e-3) / 1e9, 1),
+                     "bf16_gbs": round(sum(bb) / (b16 * 1e-3) / 1e9, 1),
+                     "bf16_speedup": round(f32 / b16, 3),
+                     "f32_kernel_ms": [round(k, 4) for k in k32], "bf16_kernel_ms": [round(k, 4) for k in k16],
+                     "rel_err_fp64_f32": e32, "rel_err_fp64_bf16": e16})
+        log(f"c3 R={R}: f32 {f32:.3f} ms, bf16 {b16:.3f} ms, err f32 {max(e32):.2e} bf16 {max(e16):.2e}")
+    r = runs[0]
+    _line("All-mode MTTKRP ms & HBM GB/s, 4-mode precision sweep", r["f32_gbs"], "GB/s", r["f32_ms"], args,
+          "f32", {"workload": f"all-mode MM-CSF MTTKRP, 100000x50000x20000x1000, {nnz} distinct nonzeros, "
+                              "Zipf1.1 x U x Zipf1.1 x U, R=16 and 64, f32 vs bf16 factors",
+                  "dims": dims, "nnz": nnz}, runs=runs, build_s=round(build_s, 2))
+
+
+def _planted_values(torch, inds, A, noise, seed, chunk=4_000_000):
+    """sum_r prod_m A_m[i_m][r] + noise * N(0,1), float64 math, float32 values."""
+    n = inds[0].numel()
+    g = torch.Generator(device=inds[0].device).manual_seed(seed)
+    out = torch.empty(n, dtype=torch.float32, device=inds[0].device)
+    for s in range(0, n, chunk):
+        e = min(s + chunk, n)
+        p = torch.ones((e - s, A[0].shape[1]), dtype=torch.float64, device=inds[0].device)
+        for m, a in enumerate(A):
+            p *= a[inds[m][s:e].long()]
+        z = torch.randn(e - s, generator=g, device=inds[0].device, dtype=torch.float64)
+        out[s:e] = (p.sum(1) + noise * z).float()
+    return out
+
+
+def _als64(torch, inds, vals, dims, U0, iters):
+    """float64 CP-ALS with the update, normalisation and fit rules of cpd()
+    (cpd_cpu.cu:484-691), on the GPU in torch, as the numerical reference."""
+    N = len(dims)
+    U = [u.double().clone() for u in U0]
+    R = U[0].shape[1]
+    ATA = [u.t() @ u for u in U]
+    xn = float((vals.double() ** 2).sum())
+    fits = []
+    lam = torch.ones(R, dtype=torch.float64, device=vals.device)
+    for it in range(iters):
+        for m in range(N):
+            M = fp64_reference(inds, vals, U, m, dims)
+            if m == N - 1:
+                Mlast = M
+            G = torch.ones((R, R), dtype=torch.float64, device=vals.device)
+            for n in range(N):
+                if n != m:
+                    G = G * ATA[n]
+            U[m] = torch.linalg.solve(G, M.t()).t()
+            lam = U[m].norm(dim=0) if it == 0 else torch.clamp(U[m].max(dim=0).values, min=1.0)
+            U[m] = U[m] / lam
+            ATA[m] = U[m].t() @ U[m]
+        H = torch.ones((R, R), dtype=torch.float64, device=vals.device)
+        for n in range(N):
+            H = H * ATA[n]
+        nm = abs(float(lam @ H @ lam))
+        inner = float(((U[N - 1] * Mlast) @ lam).sum())
+        res = xn + nm - 2 * inner
+        fits.append(1 - (res ** 0.5 if res > 0 else res) / xn ** 0.5)
+    return fits
+
+
+def run_c4(args):
+    """CP-ALS, 20 iterations at R=32 on a 200000x100000x20000 tensor with 20M
+    nonzeros sampled from a planted rank-32 |N(0,1)| model + N(0, 0.01)
+    noise (sigma 0.01); fits against a float64 ALS."""
+    import torch
+
+    import mmcsf
+
+    dev = torch.device("cuda:0")
+    dims, nnz, R, iters = [200000, 100000, 20000], args.nnz_c4, 32, 20
+    seed = 0x4D4D43534603
+    X0 = mmcsf.Tensor.generate(dims, nnz, [1, 0, 0], [1.2, 0.0, 0.0], seed)
+    inds, _ = X0.to_torch(dev)
+    del X0
+    g = torch.Generator(device=dev).manual_seed(seed)
+    A = [torch.randn((d, R), generator=g, device=dev, dtype=torch.float64).abs() for d in dims]
+    vals = _planted_values(torch, inds, A, 0.01, seed + 1)
+    X = mmcsf.Tensor.from_torch(dims, [a.int().contiguous() for a in inds], vals)
+    fmt = mmcsf.Format(X, mmcsf.TYPE_MMCSF, fbr_threshold=128)
+    U0 = [torch.rand((d, R), generator=g, device=dev) for d in dims]
+    U = [u.clone() for u in U0]
+    lam, st = mmcsf.cpd_als(fmt, U, iters=iters, tol=0.0)
+    Ub = [u.clone() for u in U0]
+    _, stb = mmcsf.cpd_als(fmt, Ub, iters=iters, tol=0.0, mttkrp_bf16=True)
+    fits64 = _als64(torch, inds, vals, dims, U0, iters)
+    diff = max(abs(a - b) for a, b in zip(st["fit_history"], fits64))
+    log(f"c4 fits gpu {st['fit_history'][-1]:.6f} fp64 {fits64[-1]:.6f} max diff {diff:.2e}")
+    ms_it = st["ms_total"] / st["iterations"]
+    _line("CP-ALS ms per iteration", round(ms_it, 3), "ms", ms_it, args, "f32",
+          {"workload": f"CP-ALS R=32, 20 iterations, 200000x100000x20000, {nnz} nonzeros, planted rank-32 "
+                       "|N(0,1)| + N(0,0.01) noise, Zipf1.2 x U x U", "dims": dims, "nnz": nnz, "R": R},
+          higher=False, fit=st["fit"], fit_history=st["fit_history"], fit_fp64=fits64,
+          max_abs_fit_diff_vs_fp64=diff, ms_mttkrp_total=st["ms_mttkrp"], lu_fallbacks=st["lu_fallbacks"],
+          bf16_mttkrp={"fit": stb["fit"], "max_abs_fit_diff_vs_fp64":
+                       max(abs(a - b) for a, b in zip(stb["fit_history"], fits64)),
+                       "ms_per_iteration": stb["ms_total"] / stb["iterations"]})
+
+
+def run_c5(args):
+    """Differentiable CP completion: 1000000x500000x100000, 50M observed
+    entries of a planted rank-16 N(0,1)/4 model + N(0, 0.01) noise, Zipf1.1 on
+    modes 0-1; 100 Adam steps (lr 1e-2) at R=16 and R=64 with float32 and
+    bfloat16 factors; gradient check against float64 central differences on a
+    10k-nonzero subsample."""
+    import torch
+
+    import mmcsf
+
+    dev = torch.device("cuda:0")
+    dims, nnz = [1000000, 500000, 100000], args.nnz_c5
+    seed = 0x4D4D43534604
+    X0 = mmcsf.Tensor.generate(dims, nnz, [1, 1, 0], [1.1, 1.1, 0.0], seed)
+    inds, _ = X0.to_torch(dev)
+    del X0
+    g = torch.Generator(device=dev).manual_seed(seed)
+    A = [torch.randn((d, 16), generator=g, device=dev, dtype=torch.float64) / 4.0 for d in dims]
+    vals = _planted_values(torch, inds, A, 0.01, seed + 1)
+    X = mmcsf.Tensor.from_torch(dims, [a.int().contiguous() for a in inds], vals)
+    fmt = mmcsf.Format(X, mmcsf.TYPE_MMCSF, fbr_threshold=128)
+    runs = []
+    for R in (16, 64):
+        g2 = torch.Generator(device=dev).manual_seed(seed + R)
+        init = [0.3 * torch.randn((d, R), generator=g2, device=dev) for d in dims]
+        for bf16 in (False, True):
+            P = [u.clone().requires_grad_(True) for u in init]
+            opt = torch.optim.Adam(P, lr=1e-2)
+            losses = []
+
+            def step():
+                opt.zero_grad(set_to_none=False)
+                F = [p.to(torch.bfloat16) for p in P] if bf16 else P
+                loss = mmcsf.cp_completion_loss(fmt, F)
+                loss.backward()
+                opt.step()
+                return loss
+
+            for _ in range(3):
+                losses.append(float(step()))
+            torch.cuda.synchronize()
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record()
+            ls = [step() for _ in range(97)]
+            e1.record()
+            torch.cuda.synchronize()
+            losses += [float(v) for v in ls]
+            ms = e0.elapsed_time(e1) / 97
+            # kernel-only loss + all-mode gradient at the final factors
+            F = [p.detach().to(torch.bfloat16) for p in P] if bf16 else [p.detach() for p in P]
+            kms = _event_ms(torch, lambda: mmcsf.cp_loss_grad(fmt, F), 5, 1)
+            runs.append({"R": R, "factors": "bf16" if bf16 else "f32", "ms_per_adam_step": round(ms, 4),
+                         "loss_grad_ms": round(kms, 4), "loss_first": losses[0], "loss_last": losses[-1],
+                         "rmse_last": (2 * losses[-1] / nnz) ** 0.5})
+            log(f"c5 R={R} {'bf16' if bf16 else 'f32'}: {ms:.3f} ms/step, loss {losses[0]:.4e} -> {losses[-1]:.4e}")
+    # gradient check on a 10k-nonzero subsample: <grad, D> vs float64 central differences
+    sel = torch.randperm(nnz, generator=torch.Generator(device=dev).manual_seed(seed + 9), device=dev)[:10000]
+    sub = [a[sel].contiguous() for a in inds]
+    sv = vals[sel].contiguous()
+    Xs = mmcsf.Tensor.from_torch(dims, [a.int() for a in sub], sv)
+    fs = mmcsf.Format(Xs, mmcsf.TYPE_MMCSF, fbr_threshold=128)
+    U = [0.3 * torch.randn((d, 16), generator=g, device=dev) for d in dims]
+    _, grads = mmcsf.cp_loss_grad(fs, U)
+    D = [torch.randn(u.shape, generator=g, device=dev, dtype=torch.float64) for u in U]
+
+    def loss64(W):
+        p = torch.ones((10000, 16), dtype=torch.float64, device=dev)
+        for m, w in enumerate(W):
+            p *= w[sub[m].long()]
+        return 0.5 * float(((p.sum(1) - sv.double()) ** 2).sum())
+
+    h = 1e-4
+    U64 = [u.double() for u in U]
+    fd = (loss64([u + h * d for u, d in zip(U64, D)]) - loss64([u - h * d for u, d in zip(U64, D)])) / (2 * h)
+    an = sum(float((gr.double() * d).sum()) for gr, d in zip(grads, D))
+    fd_rel = abs(an - fd) / abs(fd)
+    log(f"c5 FD check: analytic {an:.6e} fd {fd:.6e} rel {fd_rel:.2e}")
+    r = runs[0]
+    _line("CP completion ms per Adam step", r["ms_per_adam_step"], "ms", r["ms_per_adam_step"], args, "f32",
+          {"workload": f"CP completion, 1000000x500000x100000, {nnz} observed entries, planted rank-16 "
+                       "N(0,1)/4 + N(0,0.01), Zipf1.1 x Zipf1.1 x U, 100 Adam steps lr 1e-2",
+           "dims": dims, "nnz": nnz}, higher=False, runs=runs,
+          fd_check={"sample_nnz": 10000, "R": 16, "analytic": an, "central_difference_fp64": fd,
+                    "rel_err": fd_rel})
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--nnz", type=int, default=100_000_000)
+    ap.add_argument("--nnz-c3", type=int, default=50_000_000)
+    ap.add_argument("--nnz-c4", type=int, default=20_000_000)
+    ap.add_argument("--nnz-c5", type=int, default=50_000_000)
+    ap.add_argument("--cpu-sample", type=int, default=50_000_000)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4", "c5"],
+                    help="c2 (default): the headline all-mode MTTKRP line; c1/c3/c4/c5: the other "
+                         "configurations, one JSON line each")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: the multi-GPU path (slice-range shards per root tree) "
+              "is not built, so only --gpus 1 runs", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    torch.cuda.set_device(0)
+    {"c1": run_c1, "c2": run_c2, "c3": run_c3, "c4": run_c4, "c5": run_c5}[args.workload](args)
 
 
 if __name__ == "__main__":

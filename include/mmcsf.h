@@ -165,6 +165,8 @@ typedef struct mmcsf_format_info {
     uint64_t COOnnz, CSLnnz, HCSRnnz;  /* HB-CSF parts */
     int redunMode;             /* mttkrp.cu:315-347 */
     int modeTrees;             /* internal root trees held besides the MM-CSF tree */
+    uint64_t modeLevel[4][4];  /* MM-CSF: entries per level of the tree the MTTKRP of
+                                  mode m traverses (its root tree, else the MM-CSF tree) */
 } mmcsf_format_info;
 
 int mmcsf_format_info_get(const mmcsf_format *f, mmcsf_format_info *info);

@@ -456,6 +456,11 @@ int mmcsf_format_info_get(const mmcsf_format *fh, mmcsf_format_info *info) {
             info->droppedSlices = c->dropped;
         }
         for (int m = 0; m < f.ndims; ++m) info->modeTrees += f.hasTree[m] ? 1 : 0;
+        if (f.type == MMCSF_TYPE_MMCSF)
+            for (int m = 0; m < f.ndims; ++m) {
+                const Csf &t = f.hasTree[m] ? f.tree[m] : f.csf;
+                for (int l = 0; l < f.ndims; ++l) info->modeLevel[m][l] = t.cnt[l];
+            }
     });
 }
 

@@ -53,6 +53,7 @@ class FormatInfo(C.Structure):
         ("modeOrder", C.c_int * 4), ("nLevel", C.c_uint64 * 4), ("nFibers", C.c_uint64),
         ("droppedSlices", C.c_uint64), ("COOnnz", C.c_uint64), ("CSLnnz", C.c_uint64),
         ("HCSRnnz", C.c_uint64), ("redunMode", C.c_int), ("modeTrees", C.c_int),
+        ("modeLevel", (C.c_uint64 * 4) * 4),
     ]
 
 
@@ -158,6 +159,25 @@ class Tensor:
         return cls(h)
 
     @classmethod
+    def from_torch(cls, dims, inds, vals, stream=None):
+        """Device COO from device tensors (int32 coordinates per mode, float32
+        values); the data is copied into the handle."""
+        import torch
+        dims = np.ascontiguousarray(dims, dtype=np.uint64)
+        if len(inds) != len(dims):
+            raise ValueError("one coordinate tensor per mode")
+        for a in inds:
+            if not a.is_cuda or a.dtype != torch.int32 or not a.is_contiguous() or a.numel() != vals.numel():
+                raise ValueError("coordinates must be contiguous int32 device tensors, one entry per nonzero")
+        if not vals.is_cuda or vals.dtype != torch.float32 or not vals.is_contiguous():
+            raise ValueError("values must be a contiguous float32 device tensor")
+        ptrs = (VP * len(inds))(*[VP(a.data_ptr()) for a in inds])
+        h = VP()
+        _check(lib().mmcsf_tensor_from_device(len(dims), dims.ctypes.data_as(U64P), vals.numel(), ptrs,
+                                              VP(vals.data_ptr()), _stream_ptr(stream), C.byref(h)))
+        return cls(h)
+
+    @classmethod
     def load_tns(cls, path):
         h = VP()
         _check(lib().mmcsf_tensor_load_tns(str(path).encode(), C.byref(h)))
@@ -242,6 +262,7 @@ class Format:
             "nFibers": i.nFibers, "droppedSlices": i.droppedSlices, "COOnnz": i.COOnnz,
             "CSLnnz": i.CSLnnz, "HCSRnnz": i.HCSRnnz, "redunMode": i.redunMode,
             "modeTrees": i.modeTrees,
+            "modeLevel": [list(i.modeLevel[m][: i.ndims]) for m in range(i.ndims)],
         }
 
     def export(self, name, k=0):
