@@ -93,9 +93,23 @@ int mmcsf_options_parse(int argc, char **argv, mmcsf_options *opt);
 /* ---- tensors ---------------------------------------------------------------- */
 
 /* load_tensor, util.h:313-370: .tns text (ndims, dims, then 1-based
- * coordinates and a value per line) -> device COO with 0-based coordinates.
- * Errors (instead of exit(0)) on a missing file or ndims outside 3..4. */
+ * coordinates and a value per line) -> device COO with 0-based coordinates,
+ * in file order.  The text is parsed by up to 16 host threads.  Errors
+ * (instead of exit(0)) on a missing file, a malformed line, a coordinate
+ * outside 1..dims[m] or ndims outside 3..4. */
 int mmcsf_tensor_load_tns(const char *path, mmcsf_tensor **out);
+
+/* Writes a device COO as .tns text (1-based coordinates, the shortest decimal
+ * that reads back to the same float).  No reference counterpart (the
+ * reference only reads .tns). */
+int mmcsf_tensor_save_tns(const mmcsf_tensor *t, const char *path);
+
+/* Binary COO cache (no reference counterpart): a 56-byte header ("MMCSFCOO",
+ * version 1, ndims, nnz, dims[4], little-endian), then ndims x nnz uint32
+ * 0-based coordinates (mode-major), then nnz float values.  Loading skips
+ * text parsing entirely; the tensor equals the one that was saved. */
+int mmcsf_tensor_save_bin(const mmcsf_tensor *t, const char *path);
+int mmcsf_tensor_load_bin(const char *path, mmcsf_tensor **out);
 
 /* Device COO from host arrays in the reference layout: inds[m][x] (size_t
  * per mode, as Tensor::inds, util.h:89) and vals[x] (float). */

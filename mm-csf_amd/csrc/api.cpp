@@ -3,6 +3,8 @@
 
 #include <algorithm>
 #include <cerrno>
+#include <charconv>
+#include <thread>
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
@@ -53,6 +55,134 @@ __global__ void k_u32_check(const uint32_t *in, uint64_t n, uint64_t lim, uint32
     if (i < n && in[i] >= lim) atomicOr(bad, 1u);
 }
 
+// ---- .tns text and binary COO files ---------------------------------------
+
+void read_file(const char *path, std::string &text) {
+    FILE *fp = std::fopen(path, "rb");
+    MMCSF_REQUIRE(fp, MMCSF_ERR_IO, std::string(path) + " does not exist!");
+    std::unique_ptr<FILE, int (*)(FILE *)> guard(fp, std::fclose);
+    MMCSF_REQUIRE(std::fseek(fp, 0, SEEK_END) == 0, MMCSF_ERR_IO, "cannot seek");
+    long n = std::ftell(fp);
+    MMCSF_REQUIRE(n >= 0, MMCSF_ERR_IO, "cannot size the file");
+    std::rewind(fp);
+    text.resize((size_t)n);
+    MMCSF_REQUIRE(std::fread(text.data(), 1, (size_t)n, fp) == (size_t)n, MMCSF_ERR_IO, "read failed");
+}
+
+inline bool is_space(char c) { return c == ' ' || c == '\t' || c == '\n' || c == '\r' || c == '\v' || c == '\f'; }
+
+// unsigned decimal after optional white space; false at end of text or on a non-digit
+inline bool parse_u(const char *&p, const char *e, uint64_t &v) {
+    while (p < e && is_space(*p)) ++p;
+    auto r = std::from_chars(p, e, v);
+    if (r.ec != std::errc()) return false;
+    p = r.ptr;
+    return true;
+}
+
+inline bool parse_f(const char *&p, const char *e, float &v) {
+    while (p < e && is_space(*p)) ++p;
+    if (p < e && *p == '+') ++p;  // strtof accepts a leading '+', from_chars does not
+    auto r = std::from_chars(p, e, v);
+    if (r.ec == std::errc::result_out_of_range) {  // strtof semantics: +-inf / denormal / 0
+        char tmp[64];
+        size_t n = std::min<size_t>(sizeof tmp - 1, (size_t)(e - p));
+        std::memcpy(tmp, p, n);
+        tmp[n] = 0;
+        char *end = nullptr;
+        v = std::strtof(tmp, &end);
+        p += end - tmp;
+        return end != tmp;
+    }
+    if (r.ec != std::errc()) return false;
+    p = r.ptr;
+    return true;
+}
+
+// load_tensor's header (util.h:313-329): number of modes, then the mode lengths
+size_t parse_tns_header(const std::string &text, int &ndims, uint64_t *dims) {
+    const char *p = text.data(), *e = p + text.size();
+    uint64_t nd = 0;
+    MMCSF_REQUIRE(parse_u(p, e, nd), MMCSF_ERR_IO, "cannot read the number of modes");
+    MMCSF_REQUIRE(nd == 3 || nd == 4, MMCSF_ERR_UNSUPPORTED, "Supported tensor dimension is 3 or 4.");
+    ndims = (int)nd;
+    for (int m = 0; m < ndims; ++m) MMCSF_REQUIRE(parse_u(p, e, dims[m]), MMCSF_ERR_IO, "cannot read mode lengths");
+    return (size_t)(p - text.data());
+}
+
+// Nonzero lines (util.h:335-365): coordinates 1-based -> 0-based, value as
+// float.  The text is cut at line ends into one chunk per host thread; the
+// chunks are parsed concurrently and concatenated in file order.
+void parse_tns_body(const std::string &text, size_t begin, int ndims, const uint64_t *dims,
+                    std::vector<std::vector<uint32_t>> &inds, std::vector<float> &vals) {
+    const size_t n = text.size();
+    unsigned T = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    if (n - begin < (1u << 20)) T = 1;
+    std::vector<size_t> cut(T + 1, n);
+    cut[0] = begin;
+    for (unsigned k = 1; k < T; ++k) {
+        size_t c = std::max(cut[k - 1], begin + (n - begin) * k / T);
+        while (c < n && text[c] != '\n') ++c;
+        cut[k] = c;
+    }
+    struct Part {
+        std::vector<uint32_t> ind[kMaxDims];
+        std::vector<float> val;
+        std::string err;
+    };
+    std::vector<Part> parts(T);
+    auto work = [&](unsigned k) {
+        Part &P = parts[k];
+        const char *p = text.data() + cut[k], *e = text.data() + cut[k + 1];
+        const size_t guess = (size_t)(e - p) / (8 * ndims + 8) + 16;
+        for (int m = 0; m < ndims; ++m) P.ind[m].reserve(guess);
+        P.val.reserve(guess);
+        for (;;) {
+            uint64_t c[kMaxDims];
+            if (!parse_u(p, e, c[0])) {
+                while (p < e && is_space(*p)) ++p;
+                if (p != e) P.err = "malformed nonzero line";
+                return;
+            }
+            for (int m = 1; m < ndims; ++m)
+                if (!parse_u(p, e, c[m])) { P.err = "truncated nonzero line"; return; }
+            float v;
+            if (!parse_f(p, e, v)) { P.err = "missing value"; return; }
+            for (int m = 0; m < ndims; ++m) {
+                if (c[m] < 1 || c[m] > dims[m]) { P.err = "coordinate out of range"; return; }
+                P.ind[m].push_back((uint32_t)(c[m] - 1));
+            }
+            P.val.push_back(v);
+        }
+    };
+    std::vector<std::thread> th;
+    for (unsigned k = 1; k < T; ++k) th.emplace_back(work, k);
+    work(0);
+    for (auto &t : th) t.join();
+    size_t total = 0;
+    for (auto &P : parts) {
+        MMCSF_REQUIRE(P.err.empty(), MMCSF_ERR_IO, P.err);
+        total += P.val.size();
+    }
+    for (int m = 0; m < ndims; ++m) {
+        inds[m].clear();
+        inds[m].reserve(total);
+        for (auto &P : parts) inds[m].insert(inds[m].end(), P.ind[m].begin(), P.ind[m].end());
+    }
+    vals.clear();
+    vals.reserve(total);
+    for (auto &P : parts) vals.insert(vals.end(), P.val.begin(), P.val.end());
+}
+
+constexpr char kBinMagic[8] = {'M', 'M', 'C', 'S', 'F', 'C', 'O', 'O'};
+struct BinHeader {  // little-endian; followed by ndims x nnz uint32 coordinates, nnz float values
+    char magic[8];
+    uint32_t version;
+    uint32_t ndims;
+    uint64_t nnz;
+    uint64_t dims[kMaxDims];
+};
+
 void upload_tensor(int ndims, const uint64_t *dims, uint64_t nnz, const std::vector<std::vector<uint32_t>> &inds,
                    const float *vals, Tensor &t) {
     t.ndims = ndims;
@@ -77,6 +207,12 @@ std::vector<T> to_host(const DevArray<T> &d, uint64_t n) {
     std::vector<T> h(n);
     if (n) MMCSF_HIP(hipMemcpy(h.data(), d.p, n * sizeof(T), hipMemcpyDeviceToHost));
     return h;
+}
+
+void download_tensor(const Tensor &t, std::vector<std::vector<uint32_t>> &inds, std::vector<float> &vals) {
+    inds.resize(t.ndims);
+    for (int m = 0; m < t.ndims; ++m) inds[m] = to_host(t.inds[m], t.nnz);
+    vals = to_host(t.vals, t.nnz);
 }
 
 // Coordinates of level l for every nonzero of a CSF tree (expands the tree).
@@ -200,44 +336,100 @@ int mmcsf_options_parse(int argc, char **argv, mmcsf_options *o) {
 int mmcsf_tensor_load_tns(const char *path, mmcsf_tensor **out) {
     return guarded([&] {
         MMCSF_REQUIRE(path && out, MMCSF_ERR_ARG, "null argument");
-        std::ifstream fp(path, std::ios::binary);
-        MMCSF_REQUIRE(fp.good(), MMCSF_ERR_IO, std::string(path) + " does not exist!");
-        std::string text((std::istreambuf_iterator<char>(fp)), std::istreambuf_iterator<char>());
-        const char *p = text.c_str();
-        char *end = nullptr;
-        auto next_u = [&](uint64_t &v) -> bool {
-            errno = 0;
-            unsigned long long x = std::strtoull(p, &end, 10);
-            if (end == p) return false;
-            v = x;
-            p = end;
-            return true;
-        };
-        uint64_t nd = 0;
-        MMCSF_REQUIRE(next_u(nd), MMCSF_ERR_IO, "cannot read the number of modes");
-        MMCSF_REQUIRE(nd == 3 || nd == 4, MMCSF_ERR_UNSUPPORTED, "Supported tensor dimension is 3 or 4.");
-        const int ndims = (int)nd;
-        uint64_t dims[kMaxDims];
-        for (int m = 0; m < ndims; ++m) MMCSF_REQUIRE(next_u(dims[m]), MMCSF_ERR_IO, "cannot read mode lengths");
+        std::string text;
+        read_file(path, text);
+        int ndims = 0;
+        uint64_t dims[kMaxDims] = {0, 0, 0, 0};
+        size_t body = parse_tns_header(text, ndims, dims);
         std::vector<std::vector<uint32_t>> inds(ndims);
         std::vector<float> vals;
-        for (;;) {
-            uint64_t c[kMaxDims];
-            if (!next_u(c[0])) break;
-            for (int m = 1; m < ndims; ++m) MMCSF_REQUIRE(next_u(c[m]), MMCSF_ERR_IO, "truncated nonzero line");
-            float v = std::strtof(p, &end);
-            MMCSF_REQUIRE(end != p, MMCSF_ERR_IO, "missing value");
-            p = end;
-            for (int m = 0; m < ndims; ++m) {
-                // 1-based -> 0-based (util.h:356-360)
-                MMCSF_REQUIRE(c[m] >= 1 && c[m] <= dims[m], MMCSF_ERR_IO, "coordinate out of range");
-                inds[m].push_back((uint32_t)(c[m] - 1));
-            }
-            vals.push_back(v);
-        }
+        parse_tns_body(text, body, ndims, dims, inds, vals);
         check_shape(ndims, dims, vals.size());
         auto h = std::make_unique<mmcsf_tensor>();
         upload_tensor(ndims, dims, vals.size(), inds, vals.data(), h->t);
+        *out = h.release();
+    });
+}
+
+int mmcsf_tensor_save_tns(const mmcsf_tensor *th, const char *path) {
+    return guarded([&] {
+        MMCSF_REQUIRE(th && path, MMCSF_ERR_ARG, "null argument");
+        const Tensor &t = th->t;
+        std::vector<std::vector<uint32_t>> inds;
+        std::vector<float> vals;
+        download_tensor(t, inds, vals);
+        FILE *fp = std::fopen(path, "wb");
+        MMCSF_REQUIRE(fp, MMCSF_ERR_IO, std::string("cannot open ") + path);
+        std::string line;
+        line.reserve(128);
+        bool okw = std::fprintf(fp, "%d\n", t.ndims) > 0;
+        for (int m = 0; m < t.ndims; ++m) okw = okw && std::fprintf(fp, m ? " %llu" : "%llu", (unsigned long long)t.dims[m]) > 0;
+        okw = okw && std::fputc('\n', fp) != EOF;
+        char buf[64];
+        for (uint64_t x = 0; x < t.nnz && okw; ++x) {
+            line.clear();
+            for (int m = 0; m < t.ndims; ++m) {
+                auto r = std::to_chars(buf, buf + sizeof buf, (unsigned long long)inds[m][x] + 1);
+                line.append(buf, r.ptr);
+                line.push_back(' ');
+            }
+            // shortest representation that reads back to the same float
+            auto r = std::to_chars(buf, buf + sizeof buf, vals[x]);
+            line.append(buf, r.ptr);
+            line.push_back('\n');
+            okw = std::fwrite(line.data(), 1, line.size(), fp) == line.size();
+        }
+        okw = (std::fclose(fp) == 0) && okw;
+        MMCSF_REQUIRE(okw, MMCSF_ERR_IO, std::string("write failed: ") + path);
+    });
+}
+
+int mmcsf_tensor_save_bin(const mmcsf_tensor *th, const char *path) {
+    return guarded([&] {
+        MMCSF_REQUIRE(th && path, MMCSF_ERR_ARG, "null argument");
+        const Tensor &t = th->t;
+        std::vector<std::vector<uint32_t>> inds;
+        std::vector<float> vals;
+        download_tensor(t, inds, vals);
+        BinHeader hd{};
+        std::memcpy(hd.magic, kBinMagic, 8);
+        hd.version = 1;
+        hd.ndims = (uint32_t)t.ndims;
+        hd.nnz = t.nnz;
+        for (int m = 0; m < t.ndims; ++m) hd.dims[m] = t.dims[m];
+        FILE *fp = std::fopen(path, "wb");
+        MMCSF_REQUIRE(fp, MMCSF_ERR_IO, std::string("cannot open ") + path);
+        bool okw = std::fwrite(&hd, sizeof hd, 1, fp) == 1;
+        for (int m = 0; m < t.ndims && okw; ++m)
+            okw = std::fwrite(inds[m].data(), 4, t.nnz, fp) == t.nnz;
+        okw = okw && std::fwrite(vals.data(), 4, t.nnz, fp) == t.nnz;
+        okw = (std::fclose(fp) == 0) && okw;
+        MMCSF_REQUIRE(okw, MMCSF_ERR_IO, std::string("write failed: ") + path);
+    });
+}
+
+int mmcsf_tensor_load_bin(const char *path, mmcsf_tensor **out) {
+    return guarded([&] {
+        MMCSF_REQUIRE(path && out, MMCSF_ERR_ARG, "null argument");
+        FILE *fp = std::fopen(path, "rb");
+        MMCSF_REQUIRE(fp, MMCSF_ERR_IO, std::string(path) + " does not exist!");
+        std::unique_ptr<FILE, int (*)(FILE *)> guard(fp, std::fclose);
+        BinHeader hd{};
+        MMCSF_REQUIRE(std::fread(&hd, sizeof hd, 1, fp) == 1 && std::memcmp(hd.magic, kBinMagic, 8) == 0 &&
+                          hd.version == 1,
+                      MMCSF_ERR_IO, std::string(path) + " is not a binary COO tensor of this library");
+        MMCSF_REQUIRE(hd.ndims == 3 || hd.ndims == 4, MMCSF_ERR_UNSUPPORTED, "Supported tensor dimension is 3 or 4.");
+        const int ndims = (int)hd.ndims;
+        check_shape(ndims, hd.dims, hd.nnz);
+        std::vector<std::vector<uint32_t>> inds(ndims, std::vector<uint32_t>(hd.nnz));
+        std::vector<float> vals(hd.nnz);
+        for (int m = 0; m < ndims; ++m) {
+            MMCSF_REQUIRE(std::fread(inds[m].data(), 4, hd.nnz, fp) == hd.nnz, MMCSF_ERR_IO, "truncated file");
+            for (uint32_t c : inds[m]) MMCSF_REQUIRE(c < hd.dims[m], MMCSF_ERR_IO, "coordinate out of range");
+        }
+        MMCSF_REQUIRE(std::fread(vals.data(), 4, hd.nnz, fp) == hd.nnz, MMCSF_ERR_IO, "truncated file");
+        auto h = std::make_unique<mmcsf_tensor>();
+        upload_tensor(ndims, hd.dims, hd.nnz, inds, vals.data(), h->t);
         *out = h.release();
     });
 }

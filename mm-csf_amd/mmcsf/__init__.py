@@ -82,6 +82,9 @@ def lib():
     L.mmcsf_options_default.argtypes = [C.POINTER(Options)]
     L.mmcsf_options_parse.argtypes = [C.c_int, C.POINTER(C.c_char_p), C.POINTER(Options)]
     L.mmcsf_tensor_load_tns.argtypes = [C.c_char_p, C.POINTER(VP)]
+    L.mmcsf_tensor_load_bin.argtypes = [C.c_char_p, C.POINTER(VP)]
+    L.mmcsf_tensor_save_bin.argtypes = [VP, C.c_char_p]
+    L.mmcsf_tensor_save_tns.argtypes = [VP, C.c_char_p]
     L.mmcsf_tensor_from_host.argtypes = [C.c_int, U64P, C.c_uint64, C.POINTER(U64P),
                                          C.POINTER(C.c_float), C.POINTER(VP)]
     L.mmcsf_tensor_from_device.argtypes = [C.c_int, U64P, C.c_uint64, C.POINTER(VP), VP, VP,
@@ -176,6 +179,19 @@ class Tensor:
         _check(lib().mmcsf_tensor_from_device(len(dims), dims.ctypes.data_as(U64P), vals.numel(), ptrs,
                                               VP(vals.data_ptr()), _stream_ptr(stream), C.byref(h)))
         return cls(h)
+
+    @classmethod
+    def load_bin(cls, path):
+        """Binary COO file written by save_bin."""
+        h = VP()
+        _check(lib().mmcsf_tensor_load_bin(str(path).encode(), C.byref(h)))
+        return cls(h)
+
+    def save_bin(self, path):
+        _check(lib().mmcsf_tensor_save_bin(self._h, str(path).encode()))
+
+    def save_tns(self, path):
+        _check(lib().mmcsf_tensor_save_tns(self._h, str(path).encode()))
 
     @classmethod
     def load_tns(cls, path):
