@@ -575,18 +575,29 @@ int mmcsf_format_build(const mmcsf_tensor *th, int type, const mmcsf_options *op
             build_csf(t.ndims, t.dims, t.nnz, order, keys, std::move(vals), opt->fbrThreashold, fs, f.csf, s);
             f.csf.pos = std::move(pos);
             if (opt->modeTrees) {
-                // a root tree for every other mode: each MTTKRP becomes a root traversal
-                for (int m = 0; m < t.ndims; ++m) {
-                    if (m == order[0]) continue;
-                    int o2[kMaxDims];
+                // a root tree for every other mode: each MTTKRP becomes a root
+                // traversal.  These trees are internal, so their level order is
+                // free: 3-D trees take whichever of the two orders below the
+                // root has fewer level-1 entries (fewer fiber-level gathers).
+                auto build_tree = [&](const int *o2, Csf &c) {
                     DevArray<uint32_t> k2[kMaxDims];
                     DevArray<float> v2;
                     DevArray<uint32_t> p2;
-                    order_tensormode(t.ndims, t.dims, m, nat, o2);
                     sort_coo(t, o2, k2, v2, s, &p2);
-                    build_csf(t.ndims, t.dims, t.nnz, o2, k2, std::move(v2), opt->fbrThreashold, kNoSplit, f.tree[m],
-                              s);
-                    f.tree[m].pos = std::move(p2);
+                    build_csf(t.ndims, t.dims, t.nnz, o2, k2, std::move(v2), opt->fbrThreashold, kNoSplit, c, s);
+                    c.pos = std::move(p2);
+                };
+                for (int m = 0; m < t.ndims; ++m) {
+                    if (m == order[0]) continue;
+                    int o2[kMaxDims];
+                    order_tensormode(t.ndims, t.dims, m, nat, o2);
+                    build_tree(o2, f.tree[m]);
+                    if (t.ndims == 3) {
+                        int o3[kMaxDims] = {o2[0], o2[2], o2[1], 0};
+                        Csf alt;
+                        build_tree(o3, alt);
+                        if (alt.cnt[1] < f.tree[m].cnt[1]) f.tree[m] = std::move(alt);
+                    }
                     f.hasTree[m] = true;
                 }
             } else {

@@ -160,8 +160,10 @@ def test_adam_training_matches_float64_autograd(cuda):
         opt2.step()
         ref.append(loss.item())
     gpu, ref = np.array(gpu), np.array(ref)
-    assert ref[-1] < 0.2 * ref[0]
-    assert np.max(np.abs(gpu - ref) / ref) < 1e-3
+    assert ref[-1] < 0.5 * ref[0]  # the run trains
+    dev_ = np.max(np.abs(gpu - ref) / ref)
+    print(f"max relative deviation of the loss curve: {dev_:.3e}")
+    assert dev_ < 1e-3
 
 
 def test_error_paths(cuda):
