@@ -146,6 +146,7 @@ def run_c2(args):
     build_s = time.perf_counter() - t0
     info = fmt.info()
     log(f"built in {build_s:.2f} s: {info}")
+    log(f"per-mode tree levels: {info.get('modeLevel')}")
     inds, vals = X.to_torch(dev)
     distinct = [int((torch.bincount(inds[m].long(), minlength=dims[m]) > 0).sum()) for m in range(3)]
     g = torch.Generator(device=dev).manual_seed(SEED)

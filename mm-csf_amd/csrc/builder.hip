@@ -199,6 +199,20 @@ __global__ void k_scatter_upper(const uint32_t *__restrict__ flag, const uint32_
 
 __global__ void k_set(uint32_t *p, uint64_t i, uint32_t v) { p[i] = v; }
 
+__global__ void k_zero_words(uint32_t *p, uint64_t n) {
+    uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+// bit ptr[i] of `bits` for every entry i < n (first child of every entry)
+__global__ void k_mark_starts(const uint32_t *__restrict__ ptr, uint64_t n, uint32_t *bits) {
+    uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (i < n) {
+        uint32_t p = ptr[i];
+        atomicOr(bits + (p >> 5), 1u << (p & 31));
+    }
+}
+
 // ---------------------------------------------------------------------------
 // bins
 // ---------------------------------------------------------------------------
@@ -502,6 +516,28 @@ void build_csf(int N, const uint64_t *dims, uint64_t n, const int *order, DevArr
     c.vals = std::move(vals);
     for (int l = 0; l < N - 1; ++l) keys[l].release();
     MMCSF_HIP(hipStreamSynchronize(s));
+    build_walk_bits(c, s);
+}
+
+// ---------------------------------------------------------------------------
+// start bitmasks for the chunked root traversal: bit x of fbit = nonzero x
+// starts a leaf fiber; 4-D: bit f of sbit = leaf fiber f starts a level-1
+// entry.  Word arrays carry two spare words so 64-entry windows never read
+// past the end.
+// ---------------------------------------------------------------------------
+void build_walk_bits(Csf &c, hipStream_t s) {
+    const int N = c.ndims;
+    const uint64_t n = c.nnz, nL = c.cnt[N - 2];
+    auto mark = [&](DevArray<uint32_t> &bits, uint64_t items, const uint32_t *ptr, uint64_t nptr) {
+        const uint64_t words = items / 32 + 3;
+        bits.alloc(words);
+        hipLaunchKernelGGL(k_zero_words, dim3(nblk(words)), dim3(kBS), 0, s, bits.p, words);
+        if (nptr) hipLaunchKernelGGL(k_mark_starts, dim3(nblk(nptr)), dim3(kBS), 0, s, ptr, nptr, bits.p);
+    };
+    mark(c.fbit, n, c.ptr[N - 2].p, nL);
+    if (N == 4) mark(c.sbit, nL, c.ptr[1].p, c.cnt[1]);
+    MMCSF_HIP(hipGetLastError());
+    MMCSF_HIP(hipStreamSynchronize(s));
 }
 
 // ---------------------------------------------------------------------------
@@ -788,6 +824,7 @@ void build_hyb(int N, const uint64_t *dims, Csf &&src, Hyb &h, hipStream_t s) {
         bin_lists(cb.p, nCsl, h.cslBinList, h.cslBinCount, cdrop, s, tmp);
     }
     MMCSF_HIP(hipStreamSynchronize(s));
+    build_walk_bits(c, s);
     // the source CSF is consumed
     Csf dead = std::move(src);
     (void)dead;

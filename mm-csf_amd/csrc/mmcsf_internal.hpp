@@ -111,6 +111,8 @@ struct Csf {
     DevArray<uint32_t> pos;              // tensor-order position of every stored nonzero
                                          // (B-CSF and MM-CSF trees; empty otherwise)
     DevArray<uint32_t> fbrSlc;           // fbrLikeSlcInds: slice coord per level-1 entry
+    DevArray<uint32_t> fbit;             // bit x: nonzero x starts a leaf fiber
+    DevArray<uint32_t> sbit;             // 4-D, bit f: leaf fiber f starts a level-1 entry
     // slice bins (make_TiledBin / make_HybBin)
     bool hasBins = false;
     DevArray<uint32_t> binList;          // concatenated slcMapperBin[0..9]
@@ -162,6 +164,8 @@ void order_tensormode(int ndims, const uint64_t *dims, int mode, bool nat, int *
 
 // Sorts the tensor's nonzeros by the full key in `order`; outputs level keys
 // keys[l] = coordinate of order[l], and permuted values.
+// start bitmasks of a built tree (called by build_csf and build_hyb)
+void build_walk_bits(Csf &c, hipStream_t s);
 // pos (optional) receives the tensor-order position of every sorted nonzero.
 void sort_coo(const Tensor &t, const int *order, DevArray<uint32_t> *keys, DevArray<float> &vals,
               hipStream_t s, DevArray<uint32_t> *pos = nullptr);
@@ -196,6 +200,9 @@ void mttkrp_mode(const Format &f, int mode, const Factors &F, float *out, hipStr
 // root-level MTTKRP with one walker per lane (mttkrp_root.hip)
 bool root_walker_supports(int R);
 void mttkrp_root(const Csf &c, const Factors &F, float *out, hipStream_t s);
+// root-level MTTKRP over 64-nonzero chunks (mttkrp_chunk.hip); needs the start bitmasks
+bool chunk_supports(const Csf &c, int R);
+void mttkrp_chunk(const Csf &c, const Factors &F, float *out, hipStream_t s);
 void mttkrp_hyb(const Hyb &h, const uint64_t *dims, const Factors &F, float *out, hipStream_t s);
 void fill_zero(float *p, uint64_t n, hipStream_t s);
 

@@ -222,6 +222,11 @@ void dispatch(const Cfg &c, const RootArgs &a, hipStream_t s) {
 bool root_walker_supports(int R) { return R >= 1 && R <= 256 && R % 4 == 0; }
 
 void mttkrp_root(const Csf &c, const Factors &F, float *out, hipStream_t s) {
+    const char *e = std::getenv("MMCSF_ROOT_KERNEL");  // tuning experiment
+    if (chunk_supports(c, F.R) && !(e && e[0] == 'l')) {
+        mttkrp_chunk(c, F, out, s);
+        return;
+    }
     const int N = c.ndims;
     RootArgs a{};
     for (int l = 0; l < N - 1; ++l) {
