@@ -944,6 +944,8 @@ int mmcsf_cp_adjoint(const mmcsf_format *fh, int mode, int R, int dtype, const v
         const Csf *c = root_tree(f, mode);
         MMCSF_REQUIRE(c, MMCSF_ERR_ARG, "the format holds no tree rooted at this mode");
         MMCSF_REQUIRE(c->pos.p || f.nnz == 0, MMCSF_ERR_ARG, "format holds no nonzero positions");
+        MMCSF_REQUIRE(reinterpret_cast<uintptr_t>(grad) % 16 == 0, MMCSF_ERR_ARG,
+                      "gradient pointer is not aligned to 16 bytes");
         hipStream_t s = static_cast<hipStream_t>(stream);
         fill_zero(grad, f.dims[mode] * (uint64_t)R, s);
         cp_gradient(*c, F, weights, alpha, grad, nullptr, s);
@@ -956,9 +958,12 @@ int mmcsf_cp_loss_grad(const mmcsf_format *fh, int R, int dtype, const void *con
         MMCSF_REQUIRE(fh && grads && loss, MMCSF_ERR_ARG, "null argument");
         const Format &f = fh->f;
         const Factors F = cp_factors(f, R, dtype, factors);
-        for (int m = 0; m < f.ndims; ++m)
+        for (int m = 0; m < f.ndims; ++m) {
             MMCSF_REQUIRE(!grads[m] || root_tree(f, m), MMCSF_ERR_ARG,
                           "all-mode gradients need an MM-CSF format with a root tree per mode");
+            MMCSF_REQUIRE(reinterpret_cast<uintptr_t>(grads[m]) % 16 == 0, MMCSF_ERR_ARG,
+                          "gradient pointer is not aligned to 16 bytes");
+        }
         MMCSF_REQUIRE(f.type == MMCSF_TYPE_MMCSF || f.type == MMCSF_TYPE_BCSF, MMCSF_ERR_ARG,
                       "CP completion needs a B-CSF or MM-CSF format");
         MMCSF_REQUIRE(f.csf.pos.p || f.nnz == 0, MMCSF_ERR_ARG, "format holds no nonzero positions");

@@ -197,16 +197,14 @@ void mttkrp_coo(const Format &f, const Factors &F, float *out, hipStream_t s);
 // through the mode's root tree when the format has one
 void mttkrp_mode(const Format &f, int mode, const Factors &F, float *out, hipStream_t s,
                  hipEvent_t evStart = nullptr, hipEvent_t evEnd = nullptr);
-// root-level MTTKRP with one walker per lane (mttkrp_root.hip)
-bool root_walker_supports(int R);
+// root-level MTTKRP over 64-nonzero chunks (walk.hip); needs R % 4 == 0 and
+// the start bitmasks of the tree
+bool walk_supports(const Csf &c, int R);
 void mttkrp_root(const Csf &c, const Factors &F, float *out, hipStream_t s);
-// root-level MTTKRP over 64-nonzero chunks (mttkrp_chunk.hip); needs the start bitmasks
-bool chunk_supports(const Csf &c, int R);
-void mttkrp_chunk(const Csf &c, const Factors &F, float *out, hipStream_t s);
 void mttkrp_hyb(const Hyb &h, const uint64_t *dims, const Factors &F, float *out, hipStream_t s);
 void fill_zero(float *p, uint64_t n, hipStream_t s);
 
-// CP completion (cp.hip): loss 1/2 sum e^2 with e = model - value, residuals
+// CP completion (walk.hip): loss 1/2 sum e^2 with e = model - value, residuals
 // in tensor order, gradients of the root mode of a tree
 bool cp_supports(int R);
 void cp_forward(const Csf &c, const Factors &F, float *resid, double *loss, hipStream_t s);

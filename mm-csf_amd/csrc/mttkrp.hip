@@ -519,7 +519,7 @@ void mttkrp_csf(const Csf &c, const uint64_t *dims, int outLevel, const Factors 
     a.nnz = (uint32_t)c.nnz;
     a.R = F.R;
     a.perGroup = per_group_for(c.nnz, g.G);
-    if (outLevel == 0 && root_walker_supports(F.R)) {
+    if (outLevel == 0 && walk_supports(c, F.R)) {
         if (F.dtype == MMCSF_BF16 && F.R % 8 == 0)
             for (int l = 1; l < N; ++l) check_align(a.U[l], MMCSF_F32, g);  // 16-byte rows
         mttkrp_root(c, F, out, s);
@@ -663,7 +663,7 @@ void mttkrp_hyb(const Hyb &h, const uint64_t *dims, const Factors &F, float *out
         a.nnz = (uint32_t)c.nnz;
         a.R = F.R;
         a.perGroup = per_group_for(c.nnz, g.G);
-        if (root_walker_supports(F.R)) mttkrp_root(c, F, out, s);
+        if (walk_supports(c, F.R)) mttkrp_root(c, F, out, s);
         else if (N == 3) launch_csf_t<3, 0>(g, a, F.dtype, s);
         else launch_csf_t<4, 0>(g, a, F.dtype, s);
     }
