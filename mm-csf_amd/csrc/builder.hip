@@ -8,6 +8,8 @@
 //   create_fbrLikeSlcInds   1770-1783
 //   make_TiledBin           1887-1952
 //   create_HYB / _4D, make_HybBin  852-1142, 1786-1885
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <rocprim/rocprim.hpp>
 
@@ -202,6 +204,24 @@ __global__ void k_set(uint32_t *p, uint64_t i, uint32_t v) { p[i] = v; }
 __global__ void k_zero_words(uint32_t *p, uint64_t n) {
     uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0u;
+}
+
+// per 64-nonzero chunk: the number of slices starting in it
+__global__ void k_chunk_starts(int N, const uint32_t *__restrict__ p0, const uint32_t *__restrict__ p1,
+                               const uint32_t *__restrict__ p2, uint64_t nS, uint32_t *cnt) {
+    uint64_t s = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (s >= nS) return;
+    uint32_t x = p1[p0[s]];
+    if (N == 4) x = p2[x];
+    atomicAdd(cnt + (x >> 6), 1u);
+}
+
+// traversal cost of a chunk: 64 units, a third more when several slices
+// start in it (group-run path), 2 per slice start (row writes)
+__global__ void k_chunk_cost(const uint32_t *__restrict__ cnt, uint64_t n, bool plain, uint32_t wd, uint32_t ws,
+                             uint32_t *cost) {
+    uint64_t c = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (c < n) cost[c] = plain ? 1u : 64u + (cnt[c] >= 3 ? wd : 0u) + ws * cnt[c];
 }
 
 // bit ptr[i] of `bits` for every entry i < n (first child of every entry)
@@ -536,6 +556,27 @@ void build_walk_bits(Csf &c, hipStream_t s) {
     };
     mark(c.fbit, n, c.ptr[N - 2].p, nL);
     if (N == 4) mark(c.sbit, nL, c.ptr[1].p, c.cnt[1]);
+    // cost prefix over chunks: waves split the traversal by cost, not by
+    // nonzeros (chunks where many slices start take longer)
+    const uint64_t chunks = (n + 63) / 64;
+    c.chunkCost.alloc(chunks + 1);
+    if (chunks) {
+        Temp tmp;
+        DevArray<uint32_t> cnt(chunks), cost(chunks);
+        hipLaunchKernelGGL(k_zero_words, dim3(nblk(chunks)), dim3(kBS), 0, s, cnt.p, chunks);
+        if (c.cnt[0])
+            hipLaunchKernelGGL(k_chunk_starts, dim3(nblk(c.cnt[0])), dim3(kBS), 0, s, N, c.ptr[0].p, c.ptr[1].p,
+                               N == 4 ? c.ptr[2].p : (const uint32_t *)nullptr, c.cnt[0], cnt.p);
+        // the weighted total must fit the 32-bit prefix; else split by chunks
+        const bool plain = chunks * 96 + 2 * c.cnt[0] >= 0xF0000000ull;
+        uint32_t wd = 32, ws = 2;
+        if (const char *e = std::getenv("MMCSF_COST_W")) std::sscanf(e, "%u,%u", &wd, &ws);  // tuning experiment
+        hipLaunchKernelGGL(k_chunk_cost, dim3(nblk(chunks)), dim3(kBS), 0, s, cnt.p, chunks, plain, wd, ws, cost.p);
+        uint64_t total = exclusive_sum(cost.p, c.chunkCost.p, chunks, s, tmp);
+        hipLaunchKernelGGL(k_set, dim3(1), dim3(1), 0, s, c.chunkCost.p, chunks, (uint32_t)total);
+    } else {
+        hipLaunchKernelGGL(k_set, dim3(1), dim3(1), 0, s, c.chunkCost.p, 0, 0u);
+    }
     MMCSF_HIP(hipGetLastError());
     MMCSF_HIP(hipStreamSynchronize(s));
 }

@@ -113,6 +113,7 @@ struct Csf {
     DevArray<uint32_t> fbrSlc;           // fbrLikeSlcInds: slice coord per level-1 entry
     DevArray<uint32_t> fbit;             // bit x: nonzero x starts a leaf fiber
     DevArray<uint32_t> sbit;             // 4-D, bit f: leaf fiber f starts a level-1 entry
+    DevArray<uint32_t> chunkCost;        // exclusive prefix of the cost of 64-nonzero chunks
     // slice bins (make_TiledBin / make_HybBin)
     bool hasBins = false;
     DevArray<uint32_t> binList;          // concatenated slcMapperBin[0..9]

@@ -40,6 +40,7 @@ struct WalkArgs {
     const float *val;
     const uint32_t *fbit;  // leaf-fiber starts over nonzeros
     const uint32_t *sbit;  // 4-D: level-1 starts over leaf fibers
+    const uint32_t *cost;  // exclusive cost prefix over chunks (chunks + 1 entries)
     const uint32_t *pos;   // tensor-order position of every stored nonzero
     const float *wts;      // weights in tensor order
     const void *U[4];      // factor of each level
@@ -61,12 +62,43 @@ enum : int {
     kWeightGrad = 3,  // out += alpha * wts[pos] * product
 };
 
-__device__ __forceinline__ uint32_t upper_idx(const uint32_t *__restrict__ a, uint32_t n, uint32_t key) {
+// Largest i < n with a[i] <= key (a non-decreasing, a[0] <= key), searched by
+// the whole wave: each round probes 64 evenly spaced entries and keeps the
+// slot of the last probe <= key, so 2^26 entries take 5 rounds of one load.
+// Every lane must call it with the same arguments.
+__device__ __forceinline__ uint32_t wave_upper(const uint32_t *__restrict__ a, uint32_t n, uint32_t key) {
+    const uint32_t lane = threadIdx.x & 63;
     uint32_t lo = 0, hi = n;
     while (hi - lo > 1) {
-        uint32_t mid = lo + ((hi - lo) >> 1);
-        if (a[mid] <= key) lo = mid;
-        else hi = mid;
+        const uint32_t step = (hi - lo + 63) / 64;
+        const uint32_t probe = lo + lane * step;
+        const bool le = lane == 0 || (probe < hi && a[probe] <= key);
+        const uint64_t m = __ballot(le);
+        const uint32_t last = 63u - (uint32_t)__builtin_clzll(m);
+        lo += last * step;
+        hi = std::min(hi, lo + step);
+    }
+    return lo;
+}
+
+// first i in [0, n] with a[i] >= key (a non-decreasing over n + 1 entries)
+__device__ __forceinline__ uint32_t wave_lower(const uint32_t *__restrict__ a, uint32_t n, uint32_t key) {
+    const uint32_t lane = threadIdx.x & 63;
+    uint32_t lo = 0, hi = n + 1;  // answer in [lo, hi)
+    while (hi - lo > 1) {
+        // probes at lo + 1 + lane * step: entries >= key bound the answer above
+        const uint32_t step = (hi - lo - 1 + 63) / 64;
+        const uint32_t probe = lo + 1 + lane * step;
+        const bool lt = probe < hi && a[probe - 1] < key;  // answer > probe - 1
+        const uint64_t m = __ballot(lt);
+        if (m == 0) {
+            hi = lo + 1;
+        } else {
+            const uint32_t last = 63u - (uint32_t)__builtin_clzll(m);
+            const uint32_t nlo = lo + 1 + last * step;
+            hi = std::min(hi, nlo + step);
+            lo = nlo;
+        }
     }
     return lo;
 }
@@ -136,8 +168,11 @@ __global__ __launch_bounds__(256, (OP == kMttkrp && N == 3 && NB == 1 && G * VEC
     const uint64_t wavesPerBlock = blockDim.x >> 6;
     const uint64_t waves = (uint64_t)gridDim.x * wavesPerBlock;
     const uint64_t wv = blockIdx.x * wavesPerBlock + (threadIdx.x >> 6);
-    const uint64_t chunks = ((uint64_t)a.nnz + 63) / 64;
-    const uint64_t cb = wv * chunks / waves, ce = (wv + 1) * chunks / waves;
+    const uint32_t chunks = (uint32_t)(((uint64_t)a.nnz + 63) / 64);
+    // this wave's chunks: an equal share of the total traversal cost
+    const uint64_t total = a.cost[chunks];
+    const uint32_t cb = wave_lower(a.cost, chunks, (uint32_t)(wv * total / waves));
+    const uint32_t ce = wv + 1 == waves ? chunks : wave_lower(a.cost, chunks, (uint32_t)((wv + 1) * total / waves));
     if (cb >= ce) return;  // the whole wave leaves together
     const int R = a.R;
     const T *U0 = static_cast<const T *>(a.U[0]);
@@ -149,15 +184,14 @@ __global__ __launch_bounds__(256, (OP == kMttkrp && N == 3 && NB == 1 && G * VEC
     // leaf fiber (and 4-D level-1 entry) of the nonzero before the run; the
     // unsigned wrap at x0 = 0 stands for -1
     const uint32_t x0 = (uint32_t)(cb * 64);
-    uint32_t fcur = x0 ? upper_idx(ptrL, a.cnt[LF], x0 - 1) : kNone;
-    uint32_t jcur = kNone;
-    if constexpr (N == 4) jcur = x0 ? upper_idx(a.ptr[1], a.cnt[1], fcur) : kNone;
-    uint32_t scur;  // the open slice: its partial row is spread over acc of all groups
-    {
-        uint32_t f0 = upper_idx(ptrL, a.cnt[LF], x0);
-        if constexpr (N == 4) f0 = upper_idx(a.ptr[1], a.cnt[1], f0);
-        scur = a.slc[f0];
+    const uint32_t f0 = wave_upper(ptrL, a.cnt[LF], x0);  // leaf fiber of x0
+    uint32_t fcur = x0 == 0 ? kNone : (a.ptr[LF][f0] == x0 ? f0 - 1 : f0);
+    uint32_t jcur = kNone, j0 = f0;
+    if constexpr (N == 4) {
+        j0 = wave_upper(a.ptr[1], a.cnt[1], f0);
+        jcur = fcur == kNone ? kNone : (a.ptr[1][j0] == f0 && fcur != f0 ? j0 - 1 : j0);
     }
+    uint32_t scur = a.slc[j0];  // the open slice: its partial row is spread over acc of all groups
     // whether the open slice began inside this wave's run: then every
     // contribution to it passes through this wave, and its row is stored
     // instead of added (the output starts zeroed)
@@ -198,8 +232,8 @@ __global__ __launch_bounds__(256, (OP == kMttkrp && N == 3 && NB == 1 && G * VEC
         for (int e = 0; e < VEC; ++e) acc[b][e] = 0.f;
     };
 
-    for (uint64_t c = cb; c < ce; ++c) {
-        const uint32_t x = (uint32_t)(c * 64) + lane;
+    for (uint32_t c = cb; c < ce; ++c) {
+        const uint32_t x = c * 64 + (uint32_t)lane;
         const bool valid = x < a.nnz;
         uint32_t lw = 0, word = 0, ps = 0;
         float v = 0.f;
@@ -229,7 +263,7 @@ __global__ __launch_bounds__(256, (OP == kMttkrp && N == 3 && NB == 1 && G * VEC
         uint32_t sprev = __shfl_up(s, 1, 64);
         if (lane == 0) sprev = scur;
         const uint64_t bm = kGrad ? __ballot(s != sprev) : 0ull;  // slice starts in the chunk
-        const uint32_t nvalid = (uint32_t)std::min<uint64_t>(64, (uint64_t)a.nnz - c * 64);
+        const uint32_t nvalid = std::min<uint32_t>(64, a.nnz - c * 64);
         // Chunks with several slice starts (short slices) give every group a
         // contiguous run of entries instead of every GPR-th entry; slices are
         // then closed group by group and joined across groups by a segmented
@@ -544,6 +578,8 @@ void go(const WalkArgs &a, hipStream_t s) {
     const uint64_t chunks = ((uint64_t)a.nnz + 63) / 64;
     uint64_t blocks = (uint64_t)device_cu_count() * perCU;
     blocks = std::min(blocks, std::max<uint64_t>(1, (chunks + 3) / 4));
+    // a wave start is found by a search over the cost prefix; more waves
+    // than chunks would give empty waves only
     fn<<<dim3((unsigned)blocks), dim3(256), 0, s>>>(a);
     MMCSF_HIP(hipGetLastError());
 }
@@ -600,6 +636,7 @@ WalkArgs args_of(const Csf &c, const Factors &F) {
     a.val = c.vals.p;
     a.fbit = c.fbit.p;
     a.sbit = c.sbit.p;
+    a.cost = c.chunkCost.p;
     a.pos = c.pos.p;
     for (int l = 0; l < N; ++l) a.U[l] = F.U[c.order[l]];
     a.nnz = (uint32_t)c.nnz;
@@ -619,7 +656,8 @@ __global__ void k_half(double *p) {
 }  // namespace
 
 bool walk_supports(const Csf &c, int R) {
-    return R >= 1 && R <= 256 && R % 4 == 0 && (c.nnz == 0 || (c.fbit.p && (c.ndims == 3 || c.sbit.p)));
+    return R >= 1 && R <= 256 && R % 4 == 0 &&
+           (c.nnz == 0 || (c.fbit.p && c.chunkCost.p && (c.ndims == 3 || c.sbit.p)));
 }
 
 void mttkrp_root(const Csf &c, const Factors &F, float *out, hipStream_t s) {
