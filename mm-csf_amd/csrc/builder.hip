@@ -216,8 +216,9 @@ __global__ void k_chunk_starts(int N, const uint32_t *__restrict__ p0, const uin
     atomicAdd(cnt + (x >> 6), 1u);
 }
 
-// traversal cost of a chunk: 64 units, a third more when several slices
-// start in it (group-run path), 2 per slice start (row writes)
+// traversal cost of a chunk: 64 units, 96 more when several slices start in
+// it (group-run path and the scan that joins the runs), 8 per slice start
+// (row writes); weights from timing Zipf-rooted trees
 __global__ void k_chunk_cost(const uint32_t *__restrict__ cnt, uint64_t n, bool plain, uint32_t wd, uint32_t ws,
                              uint32_t *cost) {
     uint64_t c = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
@@ -569,7 +570,7 @@ void build_walk_bits(Csf &c, hipStream_t s) {
                                N == 4 ? c.ptr[2].p : (const uint32_t *)nullptr, c.cnt[0], cnt.p);
         // the weighted total must fit the 32-bit prefix; else split by chunks
         const bool plain = chunks * 96 + 2 * c.cnt[0] >= 0xF0000000ull;
-        uint32_t wd = 32, ws = 2;
+        uint32_t wd = 96, ws = 8;
         if (const char *e = std::getenv("MMCSF_COST_W")) std::sscanf(e, "%u,%u", &wd, &ws);  // tuning experiment
         hipLaunchKernelGGL(k_chunk_cost, dim3(nblk(chunks)), dim3(kBS), 0, s, cnt.p, chunks, plain, wd, ws, cost.p);
         uint64_t total = exclusive_sum(cost.p, c.chunkCost.p, chunks, s, tmp);
