@@ -227,7 +227,7 @@ def run_c2(args):
     achieved = bytes_f32[dom] / (f32_k[dom] * 1e-3) / 1e9
     flops = 3 * args.nnz * R * 3
     traffic = None
-    tfile = os.path.join(ROOT, "profiles", "pmc_traffic_c2.json")
+    tfile = os.path.join(ROOT, "profiles", "k_walk_c2_traffic.json")
     if os.path.exists(tfile):
         with open(tfile) as fh:
             tr = json.load(fh)
@@ -266,7 +266,7 @@ def run_c2(args):
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": f"mode {dom} MTTKRP (root lane-walker k_root over the mode's root tree), float32 factors",
+XX
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
