@@ -266,7 +266,7 @@ def run_c2(args):
         },
         "roofline": {
             "bound": "hbm",
-XX
+            "kernel": f"mode {dom} MTTKRP (chunked root walk k_walk over the mode's root tree), float32 factors",
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
