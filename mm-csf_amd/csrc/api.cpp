@@ -592,11 +592,7 @@ int mmcsf_format_build(const mmcsf_tensor *th, int type, const mmcsf_options *op
                     int o2[kMaxDims];
                     order_tensormode(t.ndims, t.dims, m, nat, o2);
                     build_tree(o2, f.tree[m]);
-                    const char *ev = std::getenv("MMCSF_TREE_ORDER");  // tuning experiment
-                    if (t.ndims == 3 && ev && ev[0] == 'a') {
-                        int o3[kMaxDims] = {o2[0], o2[2], o2[1], 0};
-                        build_tree(o3, f.tree[m]);
-                    } else if (t.ndims == 3 && !(ev && ev[0] == 'r')) {
+                    if (t.ndims == 3) {
                         int o3[kMaxDims] = {o2[0], o2[2], o2[1], 0};
                         Csf alt;
                         build_tree(o3, alt);

@@ -570,8 +570,7 @@ void build_walk_bits(Csf &c, hipStream_t s) {
                                N == 4 ? c.ptr[2].p : (const uint32_t *)nullptr, c.cnt[0], cnt.p);
         // the weighted total must fit the 32-bit prefix; else split by chunks
         const bool plain = chunks * 96 + 2 * c.cnt[0] >= 0xF0000000ull;
-        uint32_t wd = 96, ws = 8;
-        if (const char *e = std::getenv("MMCSF_COST_W")) std::sscanf(e, "%u,%u", &wd, &ws);  // tuning experiment
+        const uint32_t wd = 96, ws = 8;
         hipLaunchKernelGGL(k_chunk_cost, dim3(nblk(chunks)), dim3(kBS), 0, s, cnt.p, chunks, plain, wd, ws, cost.p);
         uint64_t total = exclusive_sum(cost.p, c.chunkCost.p, chunks, s, tmp);
         hipLaunchKernelGGL(k_set, dim3(1), dim3(1), 0, s, c.chunkCost.p, chunks, (uint32_t)total);
