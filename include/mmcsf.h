@@ -36,8 +36,12 @@ extern "C" {
 #define MMCSF_TYPE_COO_CPU 1      /* -t 1  COO on the host (sequential float) */
 #define MMCSF_TYPE_COO 3          /* -t 3  COO on the GPU */
 #define MMCSF_TYPE_BCSF 8         /* -t 8  B-CSF (tiled HCSR + slice bins) */
+#define MMCSF_TYPE_BCSF_ANYMODE 9 /* -t 9  the B-CSF of -t 8, every mode from it
+                                     (MTTKRP_B_HCSR_GPU_ANYMODE, mttkrp.cu:223-231) */
 #define MMCSF_TYPE_HBCSF 10       /* -t 10 HB-CSF (COO + CSL + CSF parts) */
 #define MMCSF_TYPE_MMCSF 12       /* -t 12 MM-CSF, every mode from one build */
+#define MMCSF_TYPE_ONECSF 14      /* -t 14 single CSF on the GPU, every mode from it
+                                     (MTTKRP_ONE_HCSR_GPU, mttkrp.cu:234-299) */
 
 /* ---- factor storage --------------------------------------------------------- */
 #define MMCSF_F32 0               /* float32 rows */
@@ -151,7 +155,9 @@ void mmcsf_tensor_free(mmcsf_tensor *t);
  * flows (mttkrp.cu):
  *   MMCSF_TYPE_COO   sort_COOtensor (util.h:412-475)
  *   MMCSF_TYPE_BCSF  create_HCSR + make_KTiling + create_TiledHCSR +
- *                    make_TiledBin + create_fbrLikeSlcInds (mttkrp.cu:153-219)
+ *                    make_TiledBin + create_fbrLikeSlcInds (mttkrp.cu:153-219);
+ *                    MMCSF_TYPE_BCSF_ANYMODE and MMCSF_TYPE_ONECSF build the
+ *                    same arrays (mttkrp.cu:153-231, 234-299)
  *   MMCSF_TYPE_HBCSF create_HCSR + create_HYB[_4D] + make_HybBin
  *                    (mttkrp.cu:138-150, util.h:852-1142, 1786-1885)
  *   MMCSF_TYPE_MMCSF mm_partition_reuseBased + populate_paritions +
@@ -220,14 +226,17 @@ void mmcsf_format_free(mmcsf_format *f);
  *   MM-CSF MTTKRP_MIHCSR_GPU[_oneMode_forCPD]
  *                                    (mttkrp_gpu.h:75-81, mttkrp_gpu.cu:3380, 3860)
  * COO, B-CSF and HB-CSF serve their root mode (modeOrder[0]) only, like
- * the reference; MM-CSF serves every mode.
+ * the reference; MM-CSF and the one-tree types 9 / 14 serve every mode
+ * (types 9 / 14: the root mode walks the tree, the other modes scatter from
+ * their level, mttkrp_gpu.cu:2131-2528, 2853-3378).
  */
 int mmcsf_mttkrp(const mmcsf_format *f, int mode, int R, int dtype,
                  const void *const *factors, float *out, void *stream);
 
 /*
- * All-mode MM-CSF MTTKRP (MTTKRP_MIHCSR_GPU's mode loop,
- * mttkrp_gpu.cu:3548-3744): outs[m] receives the MTTKRP of mode m for every
+ * All-mode MTTKRP of an MM-CSF or one-tree (types 9 / 14) format
+ * (MTTKRP_MIHCSR_GPU's mode loop, mttkrp_gpu.cu:3548-3744; the mode loops of
+ * mttkrp.cu:225-230, 244-298): outs[m] receives the MTTKRP of mode m for every
  * m in 0..ndims-1, all from the same factors.  Optional ms_per_mode (host,
  * ndims entries) receives each mode's MTTKRP kernel time (after the output
  * has been zeroed), measured with HIP events on `stream`; the call then

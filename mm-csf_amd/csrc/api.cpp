@@ -233,7 +233,8 @@ std::vector<uint32_t> expand_level(const Csf &c, int l) {
 
 bool mmcsf_type_ok(int type) {
     return type == MMCSF_TYPE_COO_CPU || type == MMCSF_TYPE_COO || type == MMCSF_TYPE_BCSF ||
-           type == MMCSF_TYPE_HBCSF || type == MMCSF_TYPE_MMCSF;
+           type == MMCSF_TYPE_BCSF_ANYMODE || type == MMCSF_TYPE_ONECSF || type == MMCSF_TYPE_HBCSF ||
+           type == MMCSF_TYPE_MMCSF;
 }
 
 int redundant_mode(int ndims, const uint64_t *dims) {
@@ -610,13 +611,17 @@ int mmcsf_format_build(const mmcsf_tensor *th, int type, const mmcsf_options *op
                 for (int l = 0; l < t.ndims; ++l) f.cooOrder[l] = order[l];
                 sort_coo(t, order, f.cooKeys, f.cooVals, s);
             } else {
+                const bool bcsf = type != MMCSF_TYPE_HBCSF;  // types 8, 9, 14
                 DevArray<uint32_t> pos;
-                sort_coo(t, order, keys, vals, s, type == MMCSF_TYPE_BCSF ? &pos : nullptr);
+                sort_coo(t, order, keys, vals, s, bcsf ? &pos : nullptr);
                 build_csf(t.ndims, t.dims, t.nnz, order, keys, std::move(vals), opt->fbrThreashold, kNoSplit,
                           f.csf, s);
-                if (type == MMCSF_TYPE_BCSF) {
+                if (bcsf) {
                     f.csf.pos = std::move(pos);
                     build_bins_tiled(f.csf, s);
+                    // one-tree all-mode types: the other modes scatter from their level
+                    if (type != MMCSF_TYPE_BCSF)
+                        for (int l = 1; l < t.ndims; ++l) build_hot_table(f.csf, l, t.dims[order[l]], f.csf.hot[l], s);
                 } else {
                     build_hyb(t.ndims, t.dims, std::move(f.csf), f.hyb, s);
                 }
@@ -659,7 +664,7 @@ int mmcsf_format_info_get(const mmcsf_format *fh, mmcsf_format_info *info) {
             info->droppedSlices = c->dropped;
         }
         for (int m = 0; m < f.ndims; ++m) info->modeTrees += f.hasTree[m] ? 1 : 0;
-        if (f.type == MMCSF_TYPE_MMCSF)
+        if (f.type == MMCSF_TYPE_MMCSF || f.type == MMCSF_TYPE_BCSF_ANYMODE || f.type == MMCSF_TYPE_ONECSF)
             for (int m = 0; m < f.ndims; ++m) {
                 const Csf &t = f.hasTree[m] ? f.tree[m] : f.csf;
                 for (int l = 0; l < f.ndims; ++l) info->modeLevel[m][l] = t.cnt[l];
@@ -791,7 +796,8 @@ int mmcsf_mttkrp_allmode(const mmcsf_format *fh, int R, int dtype, const void *c
     return guarded([&] {
         MMCSF_REQUIRE(fh && factors && outs, MMCSF_ERR_ARG, "null argument");
         const Format &f = fh->f;
-        MMCSF_REQUIRE(f.type == MMCSF_TYPE_MMCSF, MMCSF_ERR_ARG, "all-mode MTTKRP needs an MM-CSF format");
+        MMCSF_REQUIRE(f.type == MMCSF_TYPE_MMCSF || f.type == MMCSF_TYPE_BCSF_ANYMODE || f.type == MMCSF_TYPE_ONECSF,
+                      MMCSF_ERR_ARG, "all-mode MTTKRP needs an MM-CSF or one-tree (-t 9 / 14) format");
         MMCSF_REQUIRE(dtype == MMCSF_F32 || dtype == MMCSF_BF16, MMCSF_ERR_ARG, "dtype must be F32 or BF16");
         hipStream_t s = static_cast<hipStream_t>(stream);
         // one event pair per mode, around the MTTKRP kernel only
@@ -894,7 +900,9 @@ const Csf *root_tree(const Format &f, int mode) {
         if (f.hasTree[mode]) return &f.tree[mode];
         return nullptr;
     }
-    if (f.type == MMCSF_TYPE_BCSF && f.csf.order[0] == mode) return &f.csf;
+    if ((f.type == MMCSF_TYPE_BCSF || f.type == MMCSF_TYPE_BCSF_ANYMODE || f.type == MMCSF_TYPE_ONECSF) &&
+        f.csf.order[0] == mode)
+        return &f.csf;
     return nullptr;
 }
 
@@ -922,8 +930,9 @@ int mmcsf_cp_forward(const mmcsf_format *fh, int R, int dtype, const void *const
     return guarded([&] {
         MMCSF_REQUIRE(fh && loss, MMCSF_ERR_ARG, "null argument");
         const Format &f = fh->f;
-        MMCSF_REQUIRE(f.type == MMCSF_TYPE_MMCSF || f.type == MMCSF_TYPE_BCSF, MMCSF_ERR_ARG,
-                      "CP completion needs a B-CSF or MM-CSF format");
+        MMCSF_REQUIRE(f.type == MMCSF_TYPE_MMCSF || f.type == MMCSF_TYPE_BCSF || f.type == MMCSF_TYPE_BCSF_ANYMODE ||
+                          f.type == MMCSF_TYPE_ONECSF,
+                      MMCSF_ERR_ARG, "CP completion needs a B-CSF or MM-CSF format");
         const Factors F = cp_factors(f, R, dtype, factors);
         MMCSF_REQUIRE(f.csf.pos.p || f.nnz == 0, MMCSF_ERR_ARG, "format holds no nonzero positions");
         cp_forward(f.csf, F, residual, loss, static_cast<hipStream_t>(stream));
@@ -960,8 +969,9 @@ int mmcsf_cp_loss_grad(const mmcsf_format *fh, int R, int dtype, const void *con
             MMCSF_REQUIRE(reinterpret_cast<uintptr_t>(grads[m]) % 16 == 0, MMCSF_ERR_ARG,
                           "gradient pointer is not aligned to 16 bytes");
         }
-        MMCSF_REQUIRE(f.type == MMCSF_TYPE_MMCSF || f.type == MMCSF_TYPE_BCSF, MMCSF_ERR_ARG,
-                      "CP completion needs a B-CSF or MM-CSF format");
+        MMCSF_REQUIRE(f.type == MMCSF_TYPE_MMCSF || f.type == MMCSF_TYPE_BCSF || f.type == MMCSF_TYPE_BCSF_ANYMODE ||
+                          f.type == MMCSF_TYPE_ONECSF,
+                      MMCSF_ERR_ARG, "CP completion needs a B-CSF or MM-CSF format");
         MMCSF_REQUIRE(f.csf.pos.p || f.nnz == 0, MMCSF_ERR_ARG, "format holds no nonzero positions");
         hipStream_t s = static_cast<hipStream_t>(stream);
         cp_zero(loss, s);

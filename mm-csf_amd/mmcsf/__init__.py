@@ -18,11 +18,12 @@ import numpy as np
 
 __all__ = [
     "lib", "MMCSFError", "Options", "Tensor", "Format", "mttkrp", "mttkrp_allmode",
-    "TYPE_COO_CPU", "TYPE_COO", "TYPE_BCSF", "TYPE_HBCSF", "TYPE_MMCSF", "F32", "BF16",
+    "TYPE_COO_CPU", "TYPE_COO", "TYPE_BCSF", "TYPE_BCSF_ANYMODE", "TYPE_HBCSF", "TYPE_MMCSF", "TYPE_ONECSF",
+    "F32", "BF16",
     "parse_options", "to_bf16", "cpd_als", "gram",
 ]
 
-TYPE_COO_CPU, TYPE_COO, TYPE_BCSF, TYPE_HBCSF, TYPE_MMCSF = 1, 3, 8, 10, 12
+TYPE_COO_CPU, TYPE_COO, TYPE_BCSF, TYPE_BCSF_ANYMODE, TYPE_HBCSF, TYPE_MMCSF, TYPE_ONECSF = 1, 3, 8, 9, 10, 12, 14
 F32, BF16 = 0, 1
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -353,7 +354,8 @@ def mttkrp(fmt, mode, factors, out=None, stream=None):
 
 
 def mttkrp_allmode(fmt, factors, outs=None, stream=None, timed=False):
-    """All-mode MM-CSF MTTKRP; returns (outs, ms_per_mode or None)."""
+    """All-mode MTTKRP of an MM-CSF or one-tree (types 9 / 14) format;
+    returns (outs, ms_per_mode or None)."""
     import torch
     R = factors[0].shape[1]
     info = fmt.info()

@@ -219,6 +219,20 @@ int main(int argc, char **argv) {
         result.resize(dims[resultMode] * R);
         HK(hipMemcpy(result.data(), Od[resultMode], result.size() * 4, hipMemcpyDeviceToHost));
         mmcsf_format_free(F);
+    } else if (type == MMCSF_TYPE_BCSF_ANYMODE || type == MMCSF_TYPE_ONECSF) {
+        // one tree, every mode (mttkrp.cu:223-231, 234-299)
+        mmcsf_format *F = nullptr;
+        double tb = seconds();
+        CK(mmcsf_format_build(X, type, &opt, nullptr, &F));
+        if (opt.verbose) std::printf("GPU format build - time: %.3f sec \n", seconds() - tb);
+        std::vector<float> ms(ndims, 0.f);
+        CK(mmcsf_mttkrp_allmode(F, (int)R, dtype, Ud.data(), Od.data(), nullptr, ms.data()));
+        for (int m = 0; m < ndims; ++m)
+            std::printf("%s-GPU-mode %d :%g,\n", type == MMCSF_TYPE_BCSF_ANYMODE ? "ONE-B-CSF" : "singleCSF", m, ms[m]);
+        resultMode = ndims - 1;
+        result.resize(dims[resultMode] * R);
+        HK(hipMemcpy(result.data(), Od[resultMode], result.size() * 4, hipMemcpyDeviceToHost));
+        mmcsf_format_free(F);
     } else {
         std::printf("no MTTKRP\n");
     }

@@ -206,3 +206,54 @@ def test_error_paths(cuda):
     mis = big[1:].view(10, 8)
     with pytest.raises(mmcsf.MMCSFError):
         mmcsf.mttkrp(fm, 1, [mis, None, Ud[2]])
+
+
+@pytest.mark.parametrize("dims,skew", [([60, 50, 40], None), ([300, 200, 100], [1.2, 1.2, 0]),
+                                       ([30, 20, 25, 15], None), ([200, 40, 150, 8], [1.1, 0, 1.1, 0])])
+@pytest.mark.parametrize("fmt_type", [9, 14])
+def test_one_tree_types_every_mode(cuda, dims, skew, fmt_type):
+    """-t 9 / -t 14: the B-CSF arrays of -t 8, every mode computed from that one
+    tree (root walk for modeOrder[0], scatter roles for the other levels)."""
+    import torch
+    import mmcsf
+    rng = np.random.default_rng(fmt_type * 7 + len(dims))
+    inds, vals = random_tensor(rng, dims, 20000, skew)
+    t = mmcsf.Tensor.from_numpy(dims, inds, vals)
+    for mode in (0, len(dims) - 1):
+        f8 = mmcsf.Format(t, 8, mode=mode, fbr_threshold=16)
+        fo = mmcsf.Format(t, fmt_type, mode=mode, fbr_threshold=16)
+        assert fo.info()["modeOrder"] == f8.info()["modeOrder"]
+        for l in range(len(dims) - 1):
+            np.testing.assert_array_equal(fo.export("fbrPtr", l), f8.export("fbrPtr", l))
+            np.testing.assert_array_equal(fo.export("fbrIdx", l), f8.export("fbrIdx", l))
+        for b in range(10):
+            np.testing.assert_array_equal(fo.export("slcMapperBin", b), f8.export("slcMapperBin", b))
+        for R in (4, 32, 33):
+            U, Ud = factors(rng, dims, R, torch, cuda)
+            outs, _ = mmcsf.mttkrp_allmode(fo, Ud)
+            for m in range(len(dims)):
+                ref = O.mttkrp_coo_f64(inds, vals, m, U)
+                assert rel(outs[m].cpu().numpy(), ref) < 1e-5, f"type {fmt_type} mode {m} R {R}"
+                one = mmcsf.mttkrp(fo, m, Ud).cpu().numpy()
+                assert rel(one, ref) < 1e-5
+        # bfloat16 storage
+        U, Ud = factors(rng, dims, 32, torch, cuda)
+        Ub = [mmcsf.to_bf16(u) for u in Ud]
+        Ur = bf16_round(U, torch)
+        outs, _ = mmcsf.mttkrp_allmode(fo, Ub)
+        for m in range(len(dims)):
+            assert rel(outs[m].cpu().numpy(), O.mttkrp_coo_f64(inds, vals, m, Ur)) < 1e-5
+
+
+@pytest.mark.parametrize("fmt_type", [9, 14])
+def test_one_tree_cli(cuda, golden_dir, fmt_type):
+    import os
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    cli = os.path.join(root, "mm-csf_amd", "bin", "mttkrp")
+    r = subprocess.run([cli, "-i", f"{golden_dir}/toy.tns", "-t", str(fmt_type), "-R", "4", "-c", "1"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    tag = "ONE-B-CSF-GPU-mode" if fmt_type == 9 else "singleCSF-GPU-mode"
+    assert r.stdout.count(tag) == 3
+    assert "Correctness pass!" in r.stdout
