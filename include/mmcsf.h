@@ -255,6 +255,27 @@ int mmcsf_mttkrp_allmode(const mmcsf_format *f, int R, int dtype,
 int mmcsf_mttkrp_coo_host(const mmcsf_format *f, int mode, int R,
                           const float *const *factors_host, float *out_host);
 
+/* ---- TTM ---------------------------------------------------------------------- */
+
+/*
+ * Tensor-times-matrix on the leaf mode of a B-CSF-family format (types 8, 9,
+ * 14: the tree of create_HCSR with the -f split, ttm.cu:76-78) -- TTM_CPU
+ * (ttm_cpu.cpp:60-86) and TTM_GPU (ttm_gpu.cu:140-259):
+ *     Y[f][r] = sum over the nonzeros x of leaf fiber f of
+ *               v_x * U[modeOrder[ndims-1]][leaf_x][r].
+ * factor: device U of the leaf mode (dims[modeOrder[ndims-1]] x R, float32 or
+ * bfloat16 per dtype, R % 4 == 0, 16-byte aligned); Y: device float32 with
+ * one row per leaf fiber (format_info.nLevel[ndims-2] rows; the reference's
+ * semiSpTensor has one more row, util.h:3175-3180, that no fiber writes),
+ * overwritten. */
+int mmcsf_ttm(const mmcsf_format *f, int R, int dtype, const void *factor,
+              float *Y, void *stream);
+
+/* TTM_CPU (ttm_cpu.cpp:60-86): the same product on the host, sequential
+ * float32 per fiber, for the CLI's -t 1.  Host factor and output. */
+int mmcsf_ttm_host(const mmcsf_format *f, int R, const float *factor_host,
+                   float *Y_host);
+
 /* ---- CP-ALS ------------------------------------------------------------------- */
 
 typedef struct mmcsf_cpd_stats {

@@ -889,6 +889,60 @@ int mmcsf_gram(const float *U, uint64_t rows, int R, double *G, void *stream) {
     });
 }
 
+// ---- TTM ----------------------------------------------------------------------
+
+namespace {
+const Csf *ttm_tree(const Format &f) {
+    MMCSF_REQUIRE(f.type == MMCSF_TYPE_BCSF || f.type == MMCSF_TYPE_BCSF_ANYMODE || f.type == MMCSF_TYPE_ONECSF,
+                  MMCSF_ERR_ARG, "TTM needs a B-CSF format (-t 8, 9 or 14)");
+    return &f.csf;
+}
+}  // namespace
+
+int mmcsf_ttm(const mmcsf_format *fh, int R, int dtype, const void *factor, float *Y, void *stream) {
+    return guarded([&] {
+        MMCSF_REQUIRE(fh && factor && Y, MMCSF_ERR_ARG, "null argument");
+        MMCSF_REQUIRE(dtype == MMCSF_F32 || dtype == MMCSF_BF16, MMCSF_ERR_ARG, "dtype must be F32 or BF16");
+        MMCSF_REQUIRE(R >= 1 && R <= 256 && R % 4 == 0, MMCSF_ERR_UNSUPPORTED, "TTM needs R % 4 == 0 and R <= 256");
+        const Format &f = fh->f;
+        const Csf &c = *ttm_tree(f);
+        const size_t need = dtype == MMCSF_F32 || R % 8 == 0 ? 16 : 8;
+        MMCSF_REQUIRE(reinterpret_cast<uintptr_t>(factor) % need == 0 && reinterpret_cast<uintptr_t>(Y) % 16 == 0,
+                      MMCSF_ERR_ARG, "factor / output pointer is not aligned");
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        const uint64_t rows = c.cnt[c.ndims - 2];
+        fill_zero(Y, rows * (uint64_t)R, s);
+        Factors F{};
+        for (int m = 0; m < f.ndims; ++m) F.U[m] = factor;  // only the leaf factor is read
+        F.dtype = dtype;
+        F.R = R;
+        ttm_fibers(c, F, Y, s);
+    });
+}
+
+int mmcsf_ttm_host(const mmcsf_format *fh, int R, const float *U, float *Y) {
+    return guarded([&] {
+        MMCSF_REQUIRE(fh && U && Y && R >= 1, MMCSF_ERR_ARG, "bad argument");
+        const Format &f = fh->f;
+        const Csf &c = *ttm_tree(f);
+        const int N = c.ndims;
+        const uint64_t nF = c.cnt[N - 2];
+        std::vector<uint32_t> ptr = to_host(c.ptr[N - 2], nF + 1);
+        std::vector<uint32_t> leaf = to_host(c.leaf, c.nnz);
+        std::vector<float> vals = to_host(c.vals, c.nnz);
+        std::vector<float> tmp(R);
+        // one fiber at a time, summed in float, then added (ttm_cpu.cpp:68-84)
+        for (uint64_t fb = 0; fb < nF; ++fb) {
+            std::fill(tmp.begin(), tmp.end(), 0.f);
+            for (uint32_t x = ptr[fb]; x < ptr[fb + 1]; ++x) {
+                const float *u = U + (size_t)leaf[x] * R;
+                for (int r = 0; r < R; ++r) tmp[r] += vals[x] * u[r];
+            }
+            for (int r = 0; r < R; ++r) Y[fb * R + r] = tmp[r];
+        }
+    });
+}
+
 // ---- CP completion -----------------------------------------------------------
 
 namespace {

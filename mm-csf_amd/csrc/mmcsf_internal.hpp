@@ -205,6 +205,9 @@ void mttkrp_root(const Csf &c, const Factors &F, float *out, hipStream_t s);
 void mttkrp_hyb(const Hyb &h, const uint64_t *dims, const Factors &F, float *out, hipStream_t s);
 void fill_zero(float *p, uint64_t n, hipStream_t s);
 
+// TTM (walk.hip): Y[leaf fiber] = sum over its nonzeros of v * U_leaf[leaf]
+void ttm_fibers(const Csf &c, const Factors &F, float *Y, hipStream_t s);
+
 // CP completion (walk.hip): loss 1/2 sum e^2 with e = model - value, residuals
 // in tensor order, gradients of the root mode of a tree
 bool cp_supports(int R);

@@ -60,6 +60,7 @@ enum : int {
     kLossGrad = 1,    // out += alpha * e * product; sumsq += e^2
     kLossResid = 2,   // sumsq += e^2; resid[pos] = e
     kWeightGrad = 3,  // out += alpha * wts[pos] * product
+    kTtm = 4,         // out[leaf fiber] += v * U_leaf[leaf] (tensor-times-matrix)
 };
 
 // Largest i < n with a[i] <= key (a non-decreasing, a[0] <= key), searched by
@@ -156,7 +157,9 @@ __global__ __launch_bounds__(256, (OP == kMttkrp && N == 3 && NB == 1 && G * VEC
     static_assert(G % TB == 0, "steps per batch divide the steps of a chunk");
     static_assert(VEC % 4 == 0, "rows are written as float4");
     constexpr bool kModel = OP == kLossGrad || OP == kLossResid;
-    constexpr bool kGrad = OP != kLossResid;
+    constexpr bool kGrad = OP != kLossResid;  // writes rows of `out`
+    constexpr bool kUpper = OP != kTtm;       // multiplies the rows of the upper levels
+    // rows of `out` are keyed by slice, or by leaf fiber for TTM
     constexpr bool kPos = OP == kLossResid || OP == kWeightGrad;
     constexpr int GPR = 64 / G;  // rows per load instruction
     constexpr int STEPS = G;     // load instructions per chunk and column block
@@ -191,7 +194,8 @@ __global__ __launch_bounds__(256, (OP == kMttkrp && N == 3 && NB == 1 && G * VEC
         j0 = wave_upper(a.ptr[1], a.cnt[1], f0);
         jcur = fcur == kNone ? kNone : (a.ptr[1][j0] == f0 && fcur != f0 ? j0 - 1 : j0);
     }
-    uint32_t scur = a.slc[j0];  // the open slice: its partial row is spread over acc of all groups
+    // the open slice (leaf fiber for TTM): its partial row is spread over acc of all groups
+    uint32_t scur = OP == kTtm ? f0 : a.slc[j0];
     // whether the open slice began inside this wave's run: then every
     // contribution to it passes through this wave, and its row is stored
     // instead of added (the output starts zeroed)
@@ -256,9 +260,13 @@ __global__ __launch_bounds__(256, (OP == kMttkrp && N == 3 && NB == 1 && G * VEC
         }
         uint32_t i1 = 0, i2 = 0, s = kNone;
         if (valid) {
-            i1 = a.idx[1][j];
-            s = a.slc[j];
-            if constexpr (N == 4) i2 = a.idx[2][f];
+            if constexpr (kUpper) {
+                i1 = a.idx[1][j];
+                s = a.slc[j];
+                if constexpr (N == 4) i2 = a.idx[2][f];
+            } else {
+                s = f;
+            }
         }
         uint32_t sprev = __shfl_up(s, 1, 64);
         if (lane == 0) sprev = scur;
@@ -280,8 +288,8 @@ __global__ __launch_bounds__(256, (OP == kMttkrp && N == 3 && NB == 1 && G * VEC
             const int p = (int)pe(t);
             if constexpr (kGrad) sv[t] = __shfl(s, p, 64);
             li[t] = __shfl(lw, p, 64);
-            l1[t] = __shfl(i1, p, 64);
-            if constexpr (N == 4) l2[t] = __shfl(i2, p, 64);
+            if constexpr (kUpper) l1[t] = __shfl(i1, p, 64);
+            if constexpr (N == 4 && kUpper) l2[t] = __shfl(i2, p, 64);
             if constexpr (kModel) l0[t] = __shfl(valid ? s : 0u, p, 64);
             if constexpr (OP == kLossResid) pp[t] = __shfl(ps, p, 64);
             vv[t] = __shfl(v, p, 64);
@@ -317,8 +325,8 @@ __global__ __launch_bounds__(256, (OP == kMttkrp && N == 3 && NB == 1 && G * VEC
                     const int col = b * BW + k * VEC;
                     if (col < R) {
                         rw[t][b].load(UL, li[t0 + t], R, col);
-                        w1[t][b].load(U1, l1[t0 + t], R, col);
-                        if constexpr (N == 4) w2[t][b].load(U2, l2[t0 + t], R, col);
+                        if constexpr (kUpper) w1[t][b].load(U1, l1[t0 + t], R, col);
+                        if constexpr (N == 4 && kUpper) w2[t][b].load(U2, l2[t0 + t], R, col);
                         if constexpr (kModel) w0[t][b].load(U0, l0[t0 + t], R, col);
                     }
                 }
@@ -378,14 +386,17 @@ __global__ __launch_bounds__(256, (OP == kMttkrp && N == 3 && NB == 1 && G * VEC
                         float r[VEC], q[VEC];
                         if (col < R) {
                             rw[t][b].get(r);
-                            w1[t][b].get(q);
+                            if constexpr (kUpper) w1[t][b].get(q);
+                            else
+#pragma unroll
+                                for (int e = 0; e < VEC; ++e) q[e] = 1.f;
                         } else {
 #pragma unroll
                             for (int e = 0; e < VEC; ++e) r[e] = q[e] = 0.f;
                         }
 #pragma unroll
                         for (int e = 0; e < VEC; ++e) cf[t][e] = co[t] * r[e] * q[e];
-                        if constexpr (N == 4) {
+                        if constexpr (N == 4 && kUpper) {
                             if (col < R) {
                                 w2[t][b].get(q);
 #pragma unroll
@@ -696,6 +707,14 @@ void cp_gradient(const Csf &c, const Factors &F, const float *weights, float alp
         a.sumsq = loss;
         launch<kLossGrad>(c, F, a, s);
     }
+}
+
+void ttm_fibers(const Csf &c, const Factors &F, float *Y, hipStream_t s) {
+    if (c.nnz == 0) return;
+    MMCSF_REQUIRE(walk_supports(c, F.R), MMCSF_ERR_UNSUPPORTED, "TTM needs R % 4 == 0 and R <= 256");
+    WalkArgs a = args_of(c, F);
+    a.out = Y;
+    launch<kTtm>(c, F, a, s);
 }
 
 void cp_half(double *loss, hipStream_t s) {

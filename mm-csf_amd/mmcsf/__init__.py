@@ -111,6 +111,8 @@ def lib():
     L.mmcsf_gram.argtypes = [VP, C.c_uint64, C.c_int, VP, VP]
     L.mmcsf_mttkrp_coo_host.argtypes = [VP, C.c_int, C.c_int, C.POINTER(C.POINTER(C.c_float)),
                                         C.POINTER(C.c_float)]
+    L.mmcsf_ttm.argtypes = [VP, C.c_int, C.c_int, VP, VP, VP]
+    L.mmcsf_ttm_host.argtypes = [VP, C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_float)]
     L.mmcsf_cp_forward.argtypes = [VP, C.c_int, C.c_int, C.POINTER(VP), VP, VP, VP]
     L.mmcsf_cp_adjoint.argtypes = [VP, C.c_int, C.c_int, C.c_int, C.POINTER(VP), VP, C.c_float, VP, VP]
     L.mmcsf_cp_loss_grad.argtypes = [VP, C.c_int, C.c_int, C.POINTER(VP), C.c_float, C.POINTER(VP), VP, VP]
@@ -370,6 +372,36 @@ def mttkrp_allmode(fmt, factors, outs=None, stream=None, timed=False):
     _check(lib().mmcsf_mttkrp_allmode(fmt._h, int(R), _dtype_of(factors), _factor_ptrs(factors), op,
                                       _stream_ptr(stream), ms))
     return outs, ([float(v) for v in ms] if timed else None)
+
+
+def ttm(fmt, U_leaf, out=None, stream=None):
+    """Tensor-times-matrix on the leaf mode (TTM_GPU, ttm_gpu.cu:140-259) of a
+    B-CSF-family format: one float32 row of R per leaf fiber."""
+    import torch
+    info = fmt.info()
+    R = U_leaf.shape[1]
+    leaf_mode = info["modeOrder"][-1]
+    if (not U_leaf.is_cuda or not U_leaf.is_contiguous()
+            or tuple(U_leaf.shape) != (info["dims"][leaf_mode], R)):
+        raise ValueError(f"the leaf factor must be a contiguous device tensor of shape "
+                         f"({info['dims'][leaf_mode]}, {R})")
+    rows = info["nLevel"][info["ndims"] - 2]
+    if out is None:
+        out = torch.empty((rows, R), dtype=torch.float32, device=U_leaf.device)
+    _check(lib().mmcsf_ttm(fmt._h, int(R), _dtype_of([U_leaf]), VP(U_leaf.data_ptr()), VP(out.data_ptr()),
+                           _stream_ptr(stream)))
+    return out
+
+
+def ttm_host(fmt, U_leaf):
+    """TTM_CPU (ttm_cpu.cpp:60-86) on the host: numpy float32 in and out."""
+    info = fmt.info()
+    U = np.ascontiguousarray(U_leaf, dtype=np.float32)
+    R = U.shape[1]
+    Y = np.zeros((info["nLevel"][info["ndims"] - 2], R), dtype=np.float32)
+    _check(lib().mmcsf_ttm_host(fmt._h, int(R), U.ctypes.data_as(C.POINTER(C.c_float)),
+                                Y.ctypes.data_as(C.POINTER(C.c_float))))
+    return Y
 
 
 def gram(U, stream=None):

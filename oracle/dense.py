@@ -20,6 +20,9 @@ The reference's max-norm sums per-thread maxima (298-312); the definition
 (largest entry per column, clamped at 1) is used here.  ALS is invariant to
 column scaling, so the fit does not depend on this choice.
 
+ttm64 states the tensor-times-matrix of the leaf mode per leaf fiber
+(ttm_cpu.cpp:60-86).
+
 cp_loss / cp_grad state the least-squares CP loss over the observed entries,
 L = 1/2 sum_x (v_x - sum_r prod_n U_n[i_n(x)][r])^2, and its gradient,
 dL/dU_n = -MTTKRP_n(E) with E the residual tensor on the same nonzeros.
@@ -97,6 +100,18 @@ def cpd_als(inds, vals, dims, U_init, iters, tol=1e-5):
         U[m] = U[m] / nrm
         final_lam = final_lam * nrm
     return fits, U, final_lam
+
+
+def ttm64(fbr_ptr_leaf, leaf, vals, U_leaf):
+    """TTM on the leaf mode (TTM_CPU, ttm_cpu.cpp:60-86) in float64:
+    Y[f] = sum over the nonzeros x of leaf fiber f of v_x * U_leaf[leaf_x]."""
+    ptr = np.asarray(fbr_ptr_leaf, dtype=np.int64)
+    c = np.asarray(vals, dtype=np.float64)[:, None] * np.asarray(U_leaf, dtype=np.float64)[np.asarray(leaf)]
+    nf = len(ptr) - 1
+    fiber = np.repeat(np.arange(nf), np.diff(ptr))
+    Y = np.zeros((nf, c.shape[1]))
+    np.add.at(Y, fiber, c)
+    return Y
 
 
 def cp_model_values(inds, U):

@@ -257,3 +257,53 @@ def test_one_tree_cli(cuda, golden_dir, fmt_type):
     tag = "ONE-B-CSF-GPU-mode" if fmt_type == 9 else "singleCSF-GPU-mode"
     assert r.stdout.count(tag) == 3
     assert "Correctness pass!" in r.stdout
+
+
+@pytest.mark.parametrize("dims,skew", [([60, 50, 40], None), ([300, 200, 100], [1.2, 1.2, 0]),
+                                       ([30, 20, 25, 15], None)])
+@pytest.mark.parametrize("fmt_type", [8, 14])
+def test_ttm_matches_float64_and_host(cuda, dims, skew, fmt_type):
+    """TTM per leaf fiber (TTM_GPU) against the float64 restatement of TTM_CPU
+    (oracle/dense.py) on the exported tree, and against the host TTM."""
+    import torch
+    import mmcsf
+    from oracle import dense as D
+    rng = np.random.default_rng(len(dims) + fmt_type)
+    inds, vals = random_tensor(rng, dims, 20000, skew)
+    t = mmcsf.Tensor.from_numpy(dims, inds, vals)
+    for mode, fthr in ((0, 2), (len(dims) - 1, 128)):
+        fm = mmcsf.Format(t, fmt_type, mode=mode, fbr_threshold=fthr)
+        info = fm.info()
+        N = len(dims)
+        leaf_mode = info["modeOrder"][-1]
+        ptr = fm.export("fbrPtr", N - 2)
+        leaf = fm.export("inds", leaf_mode)
+        v = fm.export("vals")
+        for R in (4, 32, 64):
+            U = rng.standard_normal((dims[leaf_mode], R)).astype(np.float32)
+            ref = D.ttm64(ptr, leaf, v, U)
+            Y = mmcsf.ttm(fm, torch.from_numpy(U).to(cuda)).cpu().numpy()
+            assert Y.shape == ref.shape
+            assert rel(Y, ref) < 1e-5, f"mode {mode} R {R}"
+            Yh = mmcsf.ttm_host(fm, U)
+            assert rel(Yh, ref) < 1e-5
+        Ub = mmcsf.to_bf16(torch.from_numpy(U).to(cuda))
+        Ur = Ub.float().cpu().double().numpy()
+        assert rel(mmcsf.ttm(fm, Ub).cpu().numpy(), D.ttm64(ptr, leaf, v, Ur)) < 1e-5
+
+
+def test_ttm_cli(cuda, golden_dir, tmp_path):
+    import os
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    cli = os.path.join(root, "mm-csf_amd", "bin", "ttm")
+    out = tmp_path / "y.txt"
+    r = subprocess.run([cli, "-i", f"{golden_dir}/toy.tns", "-t", "3", "-R", "4", "-c", "1", "-o", str(out)],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert "TTM GPU - time" in r.stdout and "Correctness pass!" in r.stdout
+    lines = out.read_text().splitlines()
+    assert lines[0] == "7 x 4 semiSpTensor"  # toy.tns -m 0: seven leaf fibers
+    r = subprocess.run([cli, "-i", f"{golden_dir}/toy.tns", "-t", "1", "-R", "4"], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0 and "TTM - COO CPU time" in r.stdout

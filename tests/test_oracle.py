@@ -492,3 +492,50 @@ def test_cp_grad_matches_finite_differences():
             Um[m][i, r] -= h
             fd = (D.cp_loss(inds, vals, Up)[0] - D.cp_loss(inds, vals, Um)[0]) / (2 * h)
             np.testing.assert_allclose(g[m][i, r], fd, rtol=1e-6, atol=1e-8)
+
+
+# ---------------------------------------------------------------------------
+# TTM restatement (oracle/dense.py ttm64)
+# ---------------------------------------------------------------------------
+def test_ttm_toy_closed_form(golden_dir):
+    """Constant factors m + 0.5 (randomize_mats): every fiber row is the sum
+    of its values times (leaf mode + 0.5)."""
+    from oracle import dense as D
+    dims, inds, vals = O.load_tns(f"{golden_dir}/toy.tns")
+    b = O.build_format(8, dims, inds, vals, mode=0, fbr_threshold=128)
+    leaf_mode = b["modeOrder"][-1]
+    R = 3
+    U = np.full((dims[leaf_mode], R), leaf_mode + 0.5)
+    Y = D.ttm64(b["fbrPtr"][1], b["leaf"], b["vals"], U)
+    ptr = b["fbrPtr"][1]
+    for f in range(len(ptr) - 1):
+        s = float(np.sum(b["vals"][ptr[f]:ptr[f + 1]], dtype=np.float64))
+        np.testing.assert_allclose(Y[f], s * (leaf_mode + 0.5), rtol=1e-12)
+
+
+@pytest.mark.parametrize("dims", [[30, 20, 10], [12, 9, 8, 7]])
+def test_ttm_then_upper_levels_give_root_mttkrp(dims):
+    """MTTKRP of the root mode = per slice, the sum over its leaf fibers of
+    the TTM row times the rows of the levels above (a closed identity that ties
+    TTM to the pinned MTTKRP restatement)."""
+    from oracle import dense as D
+    rng = np.random.default_rng(len(dims))
+    inds, vals = random_tensor(rng, dims, 1500)
+    for fthr in (2, 128):
+        b = O.build_format(8, dims, inds, vals, mode=0, fbr_threshold=fthr)
+        order = b["modeOrder"]
+        N = len(dims)
+        U = [rng.standard_normal((d, 4)) for d in dims]
+        Y = D.ttm64(b["fbrPtr"][N - 2], b["leaf"], b["vals"], U[order[-1]])
+        # walk the tree: leaf fibers -> their ancestors
+        ptr, idx = b["fbrPtr"], b["fbrIdx"]
+        out = np.zeros((dims[order[0]], 4))
+        for s in range(len(idx[0])):
+            for j in range(ptr[0][s], ptr[0][s + 1]):
+                if N == 3:
+                    out[idx[0][s]] += Y[j] * U[order[1]][idx[1][j]]
+                else:
+                    for f in range(ptr[1][j], ptr[1][j + 1]):
+                        out[idx[0][s]] += Y[f] * U[order[2]][idx[2][f]] * U[order[1]][idx[1][j]]
+        ref = O.mttkrp_coo_f64(inds, vals, order[0], [u.astype(np.float32) for u in U])
+        np.testing.assert_allclose(out, ref, rtol=1e-5, atol=1e-5)
