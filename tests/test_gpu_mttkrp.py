@@ -307,3 +307,43 @@ def test_ttm_cli(cuda, golden_dir, tmp_path):
     r = subprocess.run([cli, "-i", f"{golden_dir}/toy.tns", "-t", "1", "-R", "4"], capture_output=True, text=True,
                        timeout=120)
     assert r.returncode == 0 and "TTM - COO CPU time" in r.stdout
+
+
+@pytest.mark.parametrize("fmt_type,tag", [(1, "COO CPU - time"), (3, "Loop on. COO GPU using loop - time"),
+                                          (8, "B-CSF-GPU-mode"), (10, "HYB GPU:"), (12, "Total GPU time:")])
+def test_mttkrp_cli_every_type(cuda, golden_dir, tmp_path, fmt_type, tag):
+    """The driver's stdout lines (mttkrp.cu), -c 1 against the COO CPU type and
+    the -o matrix file (write_output, util.h:3270-3283)."""
+    import os
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    cli = os.path.join(root, "mm-csf_amd", "bin", "mttkrp")
+    out = tmp_path / "m.txt"
+    args = [cli, "-i", f"{golden_dir}/toy.tns", "-t", str(fmt_type), "-R", "4", "-f", "2", "-o", str(out)]
+    if fmt_type != 1:
+        args += ["-c", "1"]
+    r = subprocess.run(args, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert tag in r.stdout
+    if fmt_type != 1:
+        assert "Correctness pass!" in r.stdout
+    head = out.read_text().splitlines()[0]
+    assert head.endswith(" x 4 matrix")
+
+
+def test_large_skewed_tensor_every_mode(cuda):
+    """2M nonzeros, Zipf(1.2) on two modes: many waves, hot slices spanning
+    waves, short-slice chunks (group-run path), cost-balanced partition."""
+    import torch
+    import mmcsf
+    rng = np.random.default_rng(123)
+    dims = [200000, 80000, 5000]
+    inds, vals = random_tensor(rng, dims, 2_000_000, [1.2, 1.2, None])
+    t = mmcsf.Tensor.from_numpy(dims, inds, vals)
+    U, Ud = factors(rng, dims, 32, torch, cuda)
+    for trees in (True, False):
+        fm = mmcsf.Format(t, 12, fbr_threshold=128, mode_trees=trees)
+        outs, _ = mmcsf.mttkrp_allmode(fm, Ud)
+        for m in range(3):
+            ref = O.mttkrp_coo_f64(inds, vals, m, U)
+            assert rel(outs[m].cpu().numpy(), ref) < 1e-5, f"mode {m} trees {trees}"
