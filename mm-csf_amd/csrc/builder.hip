@@ -437,10 +437,12 @@ void sort_coo(const Tensor &t, const int *order, DevArray<uint32_t> *keys, DevAr
 // ---------------------------------------------------------------------------
 void build_csf(int N, const uint64_t *dims, uint64_t n, const int *order, DevArray<uint32_t> *keys,
                DevArray<float> &&vals, uint64_t fthr, uint64_t fsthr, Csf &c, hipStream_t s) {
-    (void)dims;
     Temp tmp;
     c.ndims = N;
-    for (int l = 0; l < N; ++l) c.order[l] = order[l];
+    for (int l = 0; l < N; ++l) {
+        c.order[l] = order[l];
+        c.rows[l] = dims[order[l]];
+    }
     c.nnz = n;
     if (n == 0) {
         for (int l = 0; l < N - 1; ++l) {
@@ -767,7 +769,10 @@ void build_hyb(int N, const uint64_t *dims, Csf &&src, Hyb &h, hipStream_t s) {
     // for COO (4-D) and one fiber per nonzero for CSL (3-D)
     Csf &c = h.csf;
     c.ndims = N;
-    for (int l = 0; l < N; ++l) c.order[l] = src.order[l];
+    for (int l = 0; l < N; ++l) {
+        c.order[l] = src.order[l];
+        c.rows[l] = src.rows[l];
+    }
     c.nnz = n - subTot;
     c.cnt[0] = nCsf;
     if (N == 3) {

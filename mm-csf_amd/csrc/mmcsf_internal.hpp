@@ -102,6 +102,7 @@ struct HotTable {
 struct Csf {
     int ndims = 0;
     int order[kMaxDims] = {0, 1, 2, 3};  // modeOrder
+    uint64_t rows[kMaxDims] = {0, 0, 0, 0};  // dims[order[l]]: factor rows of level l
     uint64_t nnz = 0;
     uint64_t cnt[kMaxDims] = {0, 0, 0, 0};
     DevArray<uint32_t> ptr[kMaxDims - 1];

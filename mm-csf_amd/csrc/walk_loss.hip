@@ -1,0 +1,6 @@
+// Chunked root walk for the CP-completion loss and residuals.
+#include "walk_kernel.hpp"
+
+namespace mmcsf {
+template void walk_launch<kLossResid>(const Csf &, const Factors &, const WalkArgs &, hipStream_t);
+}  // namespace mmcsf
