@@ -264,7 +264,8 @@ int mmcsf_mttkrp_coo_host(const mmcsf_format *f, int mode, int R,
  *     Y[f][r] = sum over the nonzeros x of leaf fiber f of
  *               v_x * U[modeOrder[ndims-1]][leaf_x][r].
  * factor: device U of the leaf mode (dims[modeOrder[ndims-1]] x R, float32 or
- * bfloat16 per dtype, R % 4 == 0, 16-byte aligned); Y: device float32 with
+ * bfloat16 per dtype, R % 4 == 0, 16-byte aligned, the table below 4 GiB);
+ * Y: device float32 with
  * one row per leaf fiber (format_info.nLevel[ndims-2] rows; the reference's
  * semiSpTensor has one more row, util.h:3175-3180, that no fiber writes),
  * overwritten. */
@@ -315,11 +316,12 @@ int mmcsf_gram(const float *U, uint64_t rows, int R, double *G, void *stream);
  * tensor completion with a gradient optimiser:
  *     e_x = sum_r prod_n U_n[i_n(x)][r] - v_x,    L = 1/2 sum_x e_x^2,
  *     dL/dU_n = MTTKRP_n(e)  (mttkrp_cpu.cpp:60-113 with e in place of v).
- * Each call is one lane-walker pass over a CSF tree of the format, computing
+ * Each call is one chunked-walk pass over a CSF tree of the format, computing
  * e_x on the fly (float32 arithmetic, float64 loss sum).  The format must be
  * B-CSF (its root mode only) or MM-CSF (every mode when built with mode
  * trees, MMCSF_TYPE_MMCSF with modeTrees = 1).  R % 4 == 0 and R <= 128;
- * factor rows 16-byte aligned (8-byte for bfloat16 with R % 8 != 0).
+ * factor rows 16-byte aligned (8-byte for bfloat16 with R % 8 != 0); every
+ * factor table below 4 GiB (rows x R x 4 bytes), else MMCSF_ERR_UNSUPPORTED.
  * Residuals and weights are in the order of the tensor the format was built
  * from.  `loss` is a DEVICE float64 scalar, written stream-ordered. */
 
