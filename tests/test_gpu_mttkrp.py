@@ -347,3 +347,40 @@ def test_large_skewed_tensor_every_mode(cuda):
         for m in range(3):
             ref = O.mttkrp_coo_f64(inds, vals, m, U)
             assert rel(outs[m].cpu().numpy(), ref) < 1e-5, f"mode {m} trees {trees}"
+
+
+def test_factor_table_above_4gib(cuda):
+    """A 34M-row mode makes a float32 R=32 factor table of 4.35 GB, past the
+    root walk's 32-bit row offsets: MTTKRP of the other modes (whose trees
+    gather that table) takes the single-tree kernels and still matches
+    float64; CP completion refuses with MMCSF_ERR_UNSUPPORTED."""
+    import torch
+    import mmcsf
+    rng = np.random.default_rng(41)
+    dims = [34_000_000, 64, 48]
+    nnz = 200_000
+    inds, vals = random_tensor(rng, dims, nnz)
+    t = mmcsf.Tensor.from_numpy(dims, inds, vals)
+    R = 32
+    g = torch.Generator(device=cuda).manual_seed(5)
+    U = [torch.rand((d, R), generator=g, device=cuda) for d in dims]
+    ind_d = [torch.from_numpy(a).to(cuda) for a in inds]
+    val_d = torch.from_numpy(vals).to(cuda).double()
+
+    def ref(mode):
+        c = val_d[:, None].expand(-1, R).clone()
+        for m in range(3):
+            if m != mode:
+                c *= U[m][ind_d[m]].double()
+        out = torch.zeros((dims[mode], R), dtype=torch.float64, device=cuda)
+        return out.index_add_(0, ind_d[mode], c)
+
+    for mode in (1, 2):
+        f = mmcsf.Format(t, 8, mode=mode, fbr_threshold=128)
+        got = mmcsf.mttkrp(f, mode, U).double()
+        r = ref(mode)
+        assert float(torch.linalg.norm(got - r) / torch.linalg.norm(r)) < 1e-5, f"mode {mode}"
+    fm = mmcsf.Format(t, 12, fbr_threshold=128)
+    with pytest.raises(mmcsf.MMCSFError) as e:
+        mmcsf.cp_forward(fm, U)
+    assert e.value.code == -4
