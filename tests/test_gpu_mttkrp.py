@@ -384,3 +384,32 @@ def test_factor_table_above_4gib(cuda):
     with pytest.raises(mmcsf.MMCSFError) as e:
         mmcsf.cp_forward(fm, U)
     assert e.value.code == -4
+
+
+@pytest.mark.parametrize("nnz", [1, 2, 63, 64, 65, 127, 128, 129, 191, 192, 193, 640, 1000, 4099])
+def test_chunk_count_edges(cuda, nnz):
+    """The walk reads the words of chunk c+2 and the level-1 entries of chunk
+    c+1 while chunk c computes, with its chunk loop unrolled by three: tensors
+    whose nonzero counts sit at chunk multiples, and runs of one to a few
+    chunks per wave, against float64 (all modes, float32 and bfloat16, and the
+    CP loss through the same traversal)."""
+    import torch
+    import mmcsf
+    from oracle import dense as D
+    rng = np.random.default_rng(nnz)
+    dims = [37, 23, 11]
+    inds, vals = random_tensor(rng, dims, nnz, [1.3, None, None])
+    t = mmcsf.Tensor.from_numpy(dims, inds, vals)
+    for R in (4, 32):
+        U, Ud = factors(rng, dims, R, torch, cuda)
+        fm = mmcsf.Format(t, 12, fbr_threshold=128)
+        outs, _ = mmcsf.mttkrp_allmode(fm, Ud)
+        outb, _ = mmcsf.mttkrp_allmode(fm, [u.to(torch.bfloat16) for u in Ud])
+        Ub = bf16_round(U, torch)
+        for m in range(3):
+            assert rel(outs[m].cpu().numpy(), O.mttkrp_coo_f64(inds, vals, m, U)) < 1e-5, f"R {R} mode {m}"
+            assert rel(outb[m].cpu().numpy(), O.mttkrp_coo_f64(inds, vals, m, Ub)) < 1e-5, f"bf16 R {R} mode {m}"
+        U64 = [u.astype(np.float64) for u in U]
+        L64, _ = D.cp_loss(inds, vals, U64)
+        loss = mmcsf.cp_forward(fm, Ud)[0].item()
+        assert abs(loss - L64) <= 1e-5 * L64 + 1e-12, f"loss R {R}"
