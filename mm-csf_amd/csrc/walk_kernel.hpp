@@ -118,7 +118,17 @@ __global__ __launch_bounds__(256, (OP == kMttkrp && N == 3 && NB == 1 && G * VEC
     const int g = lane / G, k = lane % G;
     const uint64_t wavesPerBlock = blockDim.x >> 6;
     const uint64_t waves = (uint64_t)gridDim.x * wavesPerBlock;
-    const uint64_t wv = blockIdx.x * wavesPerBlock + (threadIdx.x >> 6);
+    // Workgroups are dispatched round-robin over the 8 XCDs (block b on XCD
+    // b % 8).  Each XCD takes 16 spans of consecutive blocks spread over the
+    // tree, so neighbouring waves (which share slices and fiber rows) sit on
+    // one L2, while every XCD still sees all parts of the tree.
+    constexpr uint32_t kSpans = 16;
+    const uint32_t spanBlocks = gridDim.x / (8u * kSpans);
+    const uint32_t lb = gridDim.x % (8u * kSpans)
+                            ? blockIdx.x
+                            : ((blockIdx.x & 7u) + 8u * ((blockIdx.x >> 3) / spanBlocks)) * spanBlocks +
+                                  (blockIdx.x >> 3) % spanBlocks;
+    const uint64_t wv = (uint64_t)lb * wavesPerBlock + (threadIdx.x >> 6);
     const uint32_t chunks = (uint32_t)(((uint64_t)a.nnz + 63) / 64);
     // this wave's chunks: an equal share of the total traversal cost
     const uint64_t total = a.cost[chunks];
