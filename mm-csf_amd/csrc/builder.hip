@@ -216,9 +216,9 @@ __global__ void k_chunk_starts(int N, const uint32_t *__restrict__ p0, const uin
     atomicAdd(cnt + (x >> 6), 1u);
 }
 
-// traversal cost of a chunk: 64 units, 96 more when several slices start in
-// it (group-run path and the scan that joins the runs), 8 per slice start
-// (row writes); weights from timing Zipf-rooted trees
+// traversal cost of a chunk: 64 units, wd more when several slices start in
+// it (group-run path and the scan that joins the runs), ws per slice start
+// (row writes); the weights (96, 24) come from timing Zipf-rooted trees
 __global__ void k_chunk_cost(const uint32_t *__restrict__ cnt, uint64_t n, bool plain, uint32_t wd, uint32_t ws,
                              uint32_t *cost) {
     uint64_t c = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
@@ -570,9 +570,11 @@ void build_walk_bits(Csf &c, hipStream_t s) {
         if (c.cnt[0])
             hipLaunchKernelGGL(k_chunk_starts, dim3(nblk(c.cnt[0])), dim3(kBS), 0, s, N, c.ptr[0].p, c.ptr[1].p,
                                N == 4 ? c.ptr[2].p : (const uint32_t *)nullptr, c.cnt[0], cnt.p);
-        // the weighted total must fit the 32-bit prefix; else split by chunks
-        const bool plain = chunks * 96 + 2 * c.cnt[0] >= 0xF0000000ull;
-        const uint32_t wd = 96, ws = 8;
+        // the weighted total (at most chunks * (64 + wd) + ws * slices, every
+        // slice starting in one chunk) must fit the 32-bit prefix; else split
+        // by chunks
+        const uint32_t wd = 96, ws = 24;
+        const bool plain = chunks * (64ull + wd) + ws * (uint64_t)c.cnt[0] >= 0xF0000000ull;
         hipLaunchKernelGGL(k_chunk_cost, dim3(nblk(chunks)), dim3(kBS), 0, s, cnt.p, chunks, plain, wd, ws, cost.p);
         uint64_t total = exclusive_sum(cost.p, c.chunkCost.p, chunks, s, tmp);
         hipLaunchKernelGGL(k_set, dim3(1), dim3(1), 0, s, c.chunkCost.p, chunks, (uint32_t)total);
