@@ -26,6 +26,8 @@
 //   * reads are software-pipelined over chunks: the level-1 entries of chunk
 //     c+1 and the words of chunk c+2 are read while chunk c's rows are in
 //     flight.
+//   * the wave order is XCD-aware: each of the 8 XCDs (which take workgroups
+//     round-robin) runs 16 spans of consecutive blocks.
 // Bound: HBM for the structure (8 bytes per nonzero, 8 per leaf fiber, 4 per
 // level-1 entry) and the L2 / MALL gather rate for factor rows.
 #pragma once

@@ -106,8 +106,10 @@ __global__ __launch_bounds__(256, (OP == kMttkrp && N == 3 && NB == 1 && G * VEC
     static_assert(VEC % 4 == 0, "rows are written as float4");
     constexpr bool kModel = OP == kLossGrad || OP == kLossResid;
     constexpr bool kGrad = OP != kLossResid;  // writes rows of `out`
-    constexpr bool kUpper = OP != kTtm;       // multiplies the rows of the upper levels
-    // rows of `out` are keyed by slice, or by leaf fiber for TTM
+    // multiplies the rows of the upper levels; rows of `out` are keyed by
+    // slice (by leaf fiber for TTM, which has no upper levels)
+    constexpr bool kUpper = OP != kTtm;
+    // residuals / weights are addressed by tensor-order position
     constexpr bool kPos = OP == kLossResid || OP == kWeightGrad;
     constexpr int GPR = 64 / G;  // rows per load instruction
     constexpr int STEPS = G;     // load instructions per chunk and column block
@@ -154,7 +156,7 @@ __global__ __launch_bounds__(256, (OP == kMttkrp && N == 3 && NB == 1 && G * VEC
     // unsigned wrap at x0 = 0 stands for -1
     const uint32_t x0 = (uint32_t)(cb * 64);
     const uint32_t f0 = wave_upper(ptrL, a.cnt[LF], x0);  // leaf fiber of x0
-    uint32_t fcur = x0 == 0 ? kNone : (a.ptr[LF][f0] == x0 ? f0 - 1 : f0);
+    const uint32_t fcur = x0 == 0 ? kNone : (a.ptr[LF][f0] == x0 ? f0 - 1 : f0);
     uint32_t jcur = kNone, j0 = f0;
     if constexpr (N == 4) {
         j0 = wave_upper(a.ptr[1], a.cnt[1], f0);
