@@ -5,9 +5,9 @@
 //   MTTKRP                 the MM-CSF kernels (float32 accumulation)
 //   Gram  A^T A            MFMA v_mfma_f32_16x16x4f32 over row chunks, chunk sums
 //                          reduced in float64
-//   Hadamard of Grams, R x R Cholesky (LU with partial pivoting when the
-//   matrix is not SPD), inverse, the row solve U = M H^-1, column norms and
-//   the fit                float64
+//   Hadamard of Grams, the R x R inverse (Gauss-Jordan in LDS for R <= 36,
+//   Cholesky above; LU with partial pivoting when the matrix is not SPD),
+//   the row solve U = M H^-1, column norms and the fit     float64
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -22,15 +22,21 @@ namespace {
 
 typedef float v4f __attribute__((ext_vector_type(4)));
 
+// LDS bytes for k_normal_inverse's three R x R float64 matrices (R <= 36)
+constexpr size_t kInverseLds = 32768;
+
 // ---------------------------------------------------------------------------
 // Gram: G += U^T U for U row-major rows x R.  blockIdx.y picks a 16x16 tile
 // pair (bi <= bj); every wave reduces `rowsPerWave` rows with MFMA:
 //   A operand (16x4)  lane l: U[k0 + l/16][16*bi + l%16]
 //   B operand (4x16)  lane l: U[k0 + l/16][16*bj + l%16]
 //   D (16x16)         lane l, reg r: D[4*(l/16) + r][l%16]
+// Four steps' operands are loaded before their MFMAs; the block's four waves
+// are summed in LDS before one wave adds the tile to G with float64 atomics.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_gram(const float *__restrict__ U, uint32_t rows, int R,
                                               uint32_t rowsPerWave, double *__restrict__ G) {
+    __shared__ float part[4][256];
     const int nb = (R + 15) / 16;
     int p = blockIdx.y, bi = 0;
     while (p >= nb - bi) {  // tile pair p -> (bi, bj), bi <= bj
@@ -38,30 +44,51 @@ __global__ __launch_bounds__(256) void k_gram(const float *__restrict__ U, uint3
         ++bi;
     }
     const int bj = bi + p;
-    const int lane = threadIdx.x & 63;
-    const uint64_t wave = (uint64_t)blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const uint64_t wave = (uint64_t)blockIdx.x * 4 + wid;
     const uint64_t r0 = wave * rowsPerWave;
-    if (r0 >= rows) return;
     const uint64_t r1 = min((uint64_t)rows, r0 + rowsPerWave);
     const int ci = 16 * bi + (lane & 15), cj = 16 * bj + (lane & 15);
+    // columns past R read column R-1 and contribute zero
+    const int cic = min(ci, R - 1), cjc = min(cj, R - 1);
+    const float mi = ci < R ? 1.f : 0.f, mj = cj < R ? 1.f : 0.f;
     const int kk = lane >> 4;
     v4f acc = {0.f, 0.f, 0.f, 0.f};
-    for (uint64_t k = r0; k < r1; k += 4) {
+    uint64_t k = r0;
+    for (; k + 16 <= r1; k += 16) {
+        float a[4], b[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const float *u = U + (k + 4 * q + kk) * (uint64_t)R;
+            a[q] = u[cic];
+            b[q] = u[cjc];
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a[q] * mi, b[q] * mj, acc, 0, 0, 0);
+    }
+    for (; k < r1; k += 4) {
         const uint64_t row = k + kk;
         float a = 0.f, b = 0.f;
         if (row < r1) {
             const float *u = U + row * (uint64_t)R;
-            if (ci < R) a = u[ci];
-            if (cj < R) b = u[cj];
+            a = u[cic] * mi;
+            b = u[cjc] * mj;
         }
         acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc, 0, 0, 0);
     }
-    for (int r = 0; r < 4; ++r) {
-        const int i = 16 * bi + 4 * (lane >> 4) + r;
-        const int j = cj;
-        if (i < R && j < R) {
-            unsafeAtomicAdd(G + (size_t)i * R + j, (double)acc[r]);
-            if (bi != bj) unsafeAtomicAdd(G + (size_t)j * R + i, (double)acc[r]);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) part[wid][lane * 4 + r] = acc[r];
+    __syncthreads();
+    if (wid == 0) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const float v = part[0][lane * 4 + r] + part[1][lane * 4 + r] + part[2][lane * 4 + r] + part[3][lane * 4 + r];
+            const int i = 16 * bi + 4 * (lane >> 4) + r;
+            const int j = cj;
+            if (i < R && j < R) {
+                unsafeAtomicAdd(G + (size_t)i * R + j, (double)v);
+                if (bi != bj) unsafeAtomicAdd(G + (size_t)j * R + i, (double)v);
+            }
         }
     }
 }
@@ -72,14 +99,21 @@ __global__ void k_zero_d(double *p, uint64_t n) {
 }
 
 // ---------------------------------------------------------------------------
-// H = Hadamard_{n != skip} G[n]; Hinv = H^-1 by Cholesky (lower), or by LU
-// with partial pivoting when a pivot is not positive (MatrixSolveNormals,
-// cpd_cpu.cu:140-208).  One workgroup; W is an R*R*3 float64 workspace.
+// H = Hadamard_{n != skip} G[n]; Hinv = H^-1, or by LU with partial pivoting
+// when a pivot is not positive (MatrixSolveNormals, cpd_cpu.cu:140-208).  One
+// workgroup; W is an R*R*3 float64 workspace.  Small R (the matrices in LDS):
+// Gauss-Jordan elimination without pivoting, all R*R entries updated in
+// parallel per step; larger R: Cholesky (lower) and the inverse of L.
 // ---------------------------------------------------------------------------
 __global__ void k_normal_inverse(const double *__restrict__ G, int N, int skip, int R, double *__restrict__ W,
                                  double *__restrict__ Hinv, int *__restrict__ status) {
     const int tid = threadIdx.x, nt = blockDim.x;
-    double *H = W, *L = W + (size_t)R * R, *X = W + 2 * (size_t)R * R;
+    // H, L and X live in LDS when the launch provides 3 R*R doubles of it
+    // (every elimination step then waits on LDS, not on L2), else in W
+    extern __shared__ double lw[];
+    const bool inLds = (size_t)3 * R * R * sizeof(double) <= kInverseLds;
+    double *const base = inLds ? lw : W;
+    double *H = base, *L = base + (size_t)R * R, *X = base + 2 * (size_t)R * R;
     __shared__ int fail;
     for (int e = tid; e < R * R; e += nt) {
         double h = 1.0;
@@ -90,8 +124,35 @@ __global__ void k_normal_inverse(const double *__restrict__ G, int N, int skip, 
     }
     if (tid == 0) fail = 0;
     __syncthreads();
+    if (inLds) {
+        constexpr int kPer = 6;  // entries per thread: R <= 36 here
+        bool ok = true;
+        for (int k = 0; k < R; ++k) {
+            const double p = H[(size_t)k * R + k];  // the same value in every thread
+            ok = p > 0.0;
+            if (!ok) break;
+            double nv[kPer];
+            int q = 0;
+            for (int e = tid; e < R * R; e += nt, ++q) {
+                const int i = e / R, j = e % R;
+                const double hik = H[(size_t)i * R + k], hkj = H[(size_t)k * R + j];
+                nv[q] = i == k ? (j == k ? 1.0 / p : hkj / p) : (j == k ? -hik / p : H[e] - hik * hkj / p);
+            }
+            __syncthreads();
+            q = 0;
+            for (int e = tid; e < R * R; e += nt, ++q) H[e] = nv[q];
+            __syncthreads();
+        }
+        if (ok) {
+            for (int e = tid; e < R * R; e += nt) Hinv[e] = H[e];
+            if (tid == 0) status[0] = 0;
+            return;
+        }
+        if (tid == 0) fail = 1;
+        __syncthreads();
+    }
     // Cholesky, column by column
-    for (int j = 0; j < R; ++j) {
+    for (int j = 0; j < R && !inLds; ++j) {
         if (tid == 0) {
             double d = H[(size_t)j * R + j];
             for (int k = 0; k < j; ++k) d -= L[(size_t)j * R + k] * L[(size_t)j * R + k];
@@ -164,7 +225,8 @@ __global__ void k_normal_inverse(const double *__restrict__ G, int N, int skip, 
     }
 }
 
-// U[i][j] = sum_k M[i][k] Hinv[k][j] (float64 accumulation)
+// U[i][j] = sum_k M[i][k] Hinv[k][j] (float64 accumulation); each thread
+// computes four columns of one row, reading M[i][k] once for all four
 __global__ __launch_bounds__(256) void k_apply_inverse(const float *__restrict__ M, const double *__restrict__ Hinv,
                                                        uint32_t rows, int R, float *__restrict__ U) {
     extern __shared__ double hs[];
@@ -174,35 +236,63 @@ __global__ __launch_bounds__(256) void k_apply_inverse(const float *__restrict__
         __syncthreads();
     }
     const double *Hp = inLds ? hs : Hinv;
-    const uint64_t total = (uint64_t)rows * R;
+    const int quads = (R + 3) / 4;
+    const uint64_t total = (uint64_t)rows * quads;
     for (uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < total;
          e += (uint64_t)gridDim.x * blockDim.x) {
-        const uint64_t i = e / R;
-        const int j = (int)(e % R);
+        const uint64_t i = e / quads;
+        const int j0 = (int)(e % quads) * 4;
         const float *m = M + i * R;
-        double s = 0.0;
-        for (int k = 0; k < R; ++k) s += (double)m[k] * Hp[(size_t)k * R + j];
-        U[e] = (float)s;
+        double s[4] = {0.0, 0.0, 0.0, 0.0};
+        if (j0 + 4 <= R) {
+            for (int k = 0; k < R; ++k) {
+                const double mk = (double)m[k];
+                const double *h = Hp + (size_t)k * R + j0;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) s[q] += mk * h[q];
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) U[i * R + j0 + q] = (float)s[q];
+        } else {
+            for (int k = 0; k < R; ++k) {
+                const double mk = (double)m[k];
+                for (int q = 0; j0 + q < R; ++q) s[q] += mk * Hp[(size_t)k * R + j0 + q];
+            }
+            for (int q = 0; j0 + q < R; ++q) U[i * R + j0 + q] = (float)s[q];
+        }
     }
 }
 
 // column 2-norms (kind 0, float64 sums of squares) or maxima (kind 1,
-// largest non-negative entry as an order-preserving uint32 bit pattern)
+// largest non-negative entry as an order-preserving uint32 bit pattern).
+// Threads t, t + R, ... of a block share column t % R; they are combined in
+// LDS first, so each block adds one value per column to the global result.
 __global__ __launch_bounds__(256) void k_colnorm(const float *__restrict__ U, uint32_t rows, int R, int kind,
                                                  double *__restrict__ sq, uint32_t *__restrict__ mx) {
+    __shared__ double ps[256];
+    __shared__ float pm[256];
     const int j = threadIdx.x % R;
     const int rowsPerPass = blockDim.x / R;
     const int sub = threadIdx.x / R;
-    if (sub >= rowsPerPass) return;
     double s = 0.0;
     float m = 0.f;
-    for (uint64_t i = (uint64_t)blockIdx.x * rowsPerPass + sub; i < rows; i += (uint64_t)gridDim.x * rowsPerPass) {
-        const float v = U[i * R + j];
-        s += (double)v * v;
-        m = fmaxf(m, v);
+    if (sub < rowsPerPass)
+        for (uint64_t i = (uint64_t)blockIdx.x * rowsPerPass + sub; i < rows; i += (uint64_t)gridDim.x * rowsPerPass) {
+            const float v = U[i * R + j];
+            s += (double)v * v;
+            m = fmaxf(m, v);
+        }
+    ps[threadIdx.x] = s;
+    pm[threadIdx.x] = m;
+    __syncthreads();
+    if (sub == 0) {
+        for (int q = 1; q < rowsPerPass; ++q) {
+            s += ps[threadIdx.x + q * R];
+            m = fmaxf(m, pm[threadIdx.x + q * R]);
+        }
+        if (kind == 0) unsafeAtomicAdd(sq + j, s);
+        else atomicMax(mx + j, __float_as_uint(m));
     }
-    if (kind == 0) unsafeAtomicAdd(sq + j, s);
-    else atomicMax(mx + j, __float_as_uint(m));
 }
 
 __global__ void k_lambda(int R, int kind, const double *__restrict__ sq, const uint32_t *__restrict__ mx,
@@ -298,9 +388,11 @@ unsigned grid_for(uint64_t n, unsigned cap = 65536) {
     return (unsigned)std::max<uint64_t>(1, std::min<uint64_t>((n + 255) / 256, cap));
 }
 
+// blocks for k_colnorm: enough to fill the chip, few enough that the
+// per-column atomics do not contend
 unsigned colnorm_grid(uint64_t rows, int R) {
     const uint64_t perPass = std::max(1, 256 / R);
-    return (unsigned)std::max<uint64_t>(1, std::min<uint64_t>((rows + perPass - 1) / perPass, 2048));
+    return (unsigned)std::max<uint64_t>(1, std::min<uint64_t>((rows + perPass - 1) / perPass, 512));
 }
 
 }  // namespace
@@ -309,7 +401,7 @@ void gram(const float *U, uint64_t rows, int R, double *G, hipStream_t s) {
     hipLaunchKernelGGL(k_zero_d, dim3(grid_for((uint64_t)R * R)), dim3(256), 0, s, G, (uint64_t)R * R);
     if (rows == 0) return;
     const int nb = (R + 15) / 16;
-    const uint32_t rowsPerWave = 1024;
+    const uint32_t rowsPerWave = 256;
     const uint64_t waves = (rows + rowsPerWave - 1) / rowsPerWave;
     dim3 grid((unsigned)((waves + 3) / 4), (unsigned)(nb * (nb + 1) / 2));
     hipLaunchKernelGGL(k_gram, grid, dim3(256), 0, s, U, (uint32_t)rows, R, rowsPerWave, G);
@@ -376,10 +468,11 @@ void cpd_als(const Format &f, int R, int iters, double tol, int dtype, float *co
     for (it = 0; it < iters; ++it) {
         for (int m = 0; m < N; ++m) {
             mttkrp_mode(f, m, F, st.M.p, s, e0, e1);
-            hipLaunchKernelGGL(k_normal_inverse, dim3(1), dim3(256), 0, s, st.G.p, N, m, R, st.W.p, st.Hinv.p,
+            const size_t ishm = 3 * RR * sizeof(double) <= kInverseLds ? 3 * RR * sizeof(double) : 0;
+            hipLaunchKernelGGL(k_normal_inverse, dim3(1), dim3(256), ishm, s, st.G.p, N, m, R, st.W.p, st.Hinv.p,
                                st.status.p);
             const size_t shm = RR * sizeof(double) <= 32768 ? RR * sizeof(double) : 0;
-            hipLaunchKernelGGL(k_apply_inverse, dim3(grid_for(f.dims[m] * R, 8192)), dim3(256), shm, s, st.M.p,
+            hipLaunchKernelGGL(k_apply_inverse, dim3(grid_for(f.dims[m] * ((R + 3) / 4), 8192)), dim3(256), shm, s, st.M.p,
                                st.Hinv.p, (uint32_t)f.dims[m], R, U[m]);
             const int kind = it == 0 ? 0 : 1;  // 2-norm at iteration 0, max-norm after
             MMCSF_HIP(hipMemsetAsync(st.sq.p, 0, R * sizeof(double), s));

@@ -41,7 +41,9 @@ def planted_tensor(rng, dims, nnz, rank, zipf0=1.2, noise=0.01):
     return inds, vals.astype(np.float32)
 
 
-@pytest.mark.parametrize("R", [8, 16])
+# R <= 36 inverts the normal matrix by Gauss-Jordan in LDS, R = 48 by the
+# Cholesky path; R = 5 has a partial column quad in the row solve
+@pytest.mark.parametrize("R", [5, 8, 16, 48])
 def test_cpd_als_matches_float64_als(cuda, R):
     import torch
     import mmcsf
