@@ -358,6 +358,101 @@ def run_c1(args):
           t1_cpu={"ms": round(t1_s * 1e3, 2), "cores": 1, "rel_err_fp64": t1_err})
 
 
+def run_next(args):
+    """The rows beyond the five MTTKRP types, each timed with its parity check:
+    (1) the parallel .tns reader and the binary COO cache against the
+    single-threaded reader of the oracle (load_tensor, util.h:313-370);
+    (2) the one-tree all-mode types -t 9 / -t 14 against float64, beside the
+    default -t 12 (a root tree per mode); (3) leaf-mode TTM (ttm.cu) against
+    float64 and the sequential host TTM (TTM_CPU, ttm_cpu.cpp:60-86)."""
+    import tempfile
+
+    import numpy as np
+    import torch
+
+    import mmcsf
+    import oracle as O  # the single-threaded reader restated
+
+    dev = torch.device("cuda:0")
+    # (1) text reader and binary cache: 2000^3 uniform, 2M nonzeros
+    dims1 = [2000, 2000, 2000]
+    X1 = mmcsf.Tensor.generate(dims1, 2_000_000, [0, 0, 0], [0.0, 0.0, 0.0], 0x4D4D43534600)
+    _, gi, gv = X1.to_numpy()
+    with tempfile.TemporaryDirectory() as d:
+        p_tns, p_bin = os.path.join(d, "x.tns"), os.path.join(d, "x.bin")
+        X1.save_tns(p_tns)
+        X1.save_bin(p_bin)
+        tns_mb = os.path.getsize(p_tns) / 1e6
+        t0 = time.perf_counter()
+        Y = mmcsf.Tensor.load_tns(p_tns)
+        par_s = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        ref = O.load_tns(p_tns)
+        ora_s = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        Z = mmcsf.Tensor.load_bin(p_bin)
+        bin_s = time.perf_counter() - t0
+    _, yi, yv = Y.to_numpy()
+    _, zi, zv = Z.to_numpy()
+    same = all(np.array_equal(yi[m], ref[1][m]) and np.array_equal(zi[m], gi[m]) for m in range(3)) and \
+        np.array_equal(yv, ref[2]) and np.array_equal(zv.view(np.uint32), gv.view(np.uint32))
+    io = {"tns_MB": round(tns_mb, 1), "reader_s": round(par_s, 3), "reader_MB_s": round(tns_mb / par_s, 1),
+          "oracle_reader_s": round(ora_s, 3), "speedup": round(ora_s / par_s, 1), "bin_load_s": round(bin_s, 3),
+          "identical": bool(same)}
+    log(f"next io: {io}")
+    del X1, Y, Z, gi, gv, yi, yv, zi, zv, ref
+
+    # (2) one-tree types on the c2 shape with fewer nonzeros
+    dims, R = [500000, 200000, 50000], 32
+    X = mmcsf.Tensor.generate(dims, args.nnz_next, [1, 1, 0], [1.2, 1.2, 0.0], SEED)
+    inds, vals = X.to_torch(dev)
+    g = torch.Generator(device=dev).manual_seed(SEED)
+    U = [torch.rand((d, R), generator=g, device=dev) for d in dims]
+    refs = [fp64_reference(inds, vals, U, m, dims) for m in range(3)]
+    outs = [torch.empty((d, R), dtype=torch.float32, device=dev) for d in dims]
+    types = {}
+    for typ, name in ((12, "MM-CSF, a root tree per mode"), (9, "ONE-B-CSF"), (14, "single CSF")):
+        fmt = mmcsf.Format(X, typ, fbr_threshold=128)
+        ms = _event_ms(torch, lambda: mmcsf.mttkrp_allmode(fmt, U, outs), args.steps, args.warmup)
+        errs = [rel_err(outs[m], refs[m])[0] for m in range(3)]
+        types[str(typ)] = {"name": name, "all_mode_ms": round(ms, 4), "rel_err_fp64": errs}
+        log(f"next -t {typ}: {ms:.3f} ms, errs {errs}")
+        del fmt
+    del refs
+
+    # (3) TTM on the leaf mode of the B-CSF tree rooted at mode 0
+    fmt = mmcsf.Format(X, 8, mode=0, fbr_threshold=128)
+    info = fmt.info()
+    leaf_mode = info["modeOrder"][-1]
+    ptr = torch.from_numpy(fmt.export("fbrPtr", 1).astype(np.int64)).to(dev)
+    leaf = torch.from_numpy(fmt.export("inds", leaf_mode).astype(np.int64)).to(dev)
+    v = torch.from_numpy(fmt.export("vals")).to(dev).double()
+    nf = ptr.numel() - 1
+    fid = torch.repeat_interleave(torch.arange(nf, device=dev), ptr[1:] - ptr[:-1])
+    Y64 = torch.zeros((nf, R), dtype=torch.float64, device=dev)
+    Ul = U[leaf_mode]
+    for s0 in range(0, v.numel(), 4_000_000):
+        sl = slice(s0, s0 + 4_000_000)
+        Y64.index_add_(0, fid[sl], v[sl, None] * Ul[leaf[sl]].double())
+    Yd = torch.empty((nf, R), dtype=torch.float32, device=dev)
+    ms = _event_ms(torch, lambda: mmcsf.ttm(fmt, Ul, Yd), args.steps, args.warmup)
+    err = float(torch.linalg.norm(Yd.double() - Y64) / torch.linalg.norm(Y64))
+    t0 = time.perf_counter()
+    Yh = mmcsf.ttm_host(fmt, Ul.cpu().numpy())
+    host_s = time.perf_counter() - t0
+    errh = float(torch.linalg.norm(torch.from_numpy(Yh).to(dev).double() - Y64) / torch.linalg.norm(Y64))
+    nnz = int(v.numel())
+    ttm = {"leaf_fibers": nf, "gpu_ms": round(ms, 4), "rel_err_fp64": err, "host_ttm_ms": round(host_s * 1e3, 1),
+           "host_cores": 1, "host_rel_err_fp64": errh,
+           "gbs": round((nnz * 8 + nf * 8 + nf * R * 4) / (ms * 1e-3) / 1e9, 1)}
+    log(f"next ttm: {ttm}")
+    _line("Rows beyond the MTTKRP types: .tns reader, one-tree types, TTM", types["9"]["all_mode_ms"], "ms",
+          types["9"]["all_mode_ms"], args, "f32",
+          {"workload": f"reader: 2000^3 uniform 2M nonzeros; one-tree types and TTM: 500000x200000x50000, "
+                       f"{args.nnz_next} nonzeros, Zipf1.2 x Zipf1.2 x uniform, R=32, -f 128",
+           "R": R, "nnz": args.nnz_next}, higher=False, io=io, types=types, ttm=ttm)
+
+
 def run_c3(args):
     """4-mode precision sweep: 100000x50000x20000x1000, 50M nonzeros, Zipf1.1 on
     modes 0 and 2; all-mode MTTKRP at R=16 and R=64, float32 vs bfloat16
@@ -593,13 +688,14 @@ def main():
     ap.add_argument("--nnz-c3", type=int, default=50_000_000)
     ap.add_argument("--nnz-c4", type=int, default=20_000_000)
     ap.add_argument("--nnz-c5", type=int, default=50_000_000)
+    ap.add_argument("--nnz-next", type=int, default=20_000_000)
     ap.add_argument("--cpu-sample", type=int, default=50_000_000)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--profile", action="store_true",
                     help="c2: only the timed MTTKRP launches and a 1 GiB copy (for rocprofv3 runs)")
-    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4", "c5"],
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4", "c5", "next"],
                     help="c2 (default): the headline all-mode MTTKRP line; c1/c3/c4/c5: the other "
-                         "configurations, one JSON line each")
+                         "configurations; next: the .tns reader, one-tree types and TTM; one JSON line each")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the multi-GPU path (slice-range shards per root tree) "
@@ -607,7 +703,8 @@ def main():
         sys.exit(2)
     import torch
     torch.cuda.set_device(0)
-    {"c1": run_c1, "c2": run_c2, "c3": run_c3, "c4": run_c4, "c5": run_c5}[args.workload](args)
+    {"c1": run_c1, "c2": run_c2, "c3": run_c3, "c4": run_c4, "c5": run_c5,
+     "next": run_next}[args.workload](args)
 
 
 if __name__ == "__main__":
