@@ -63,6 +63,14 @@ def lib():
                                          C.c_uint64, C.POINTER(F32P), F32P]
         L.orc_mttkrp_coo_f64.argtypes = [C.c_int, C.c_uint64, C.POINTER(U64P), F32P, C.c_int,
                                          C.c_uint64, C.POINTER(F32P), C.POINTER(C.c_double)]
+        U32PP = C.POINTER(C.POINTER(C.c_uint32))
+        D64P = C.POINTER(C.c_double)
+        L.orc_mttkrp_f64_mt.argtypes = [C.c_int, C.c_uint64, U32PP, F32P, D64P, C.c_int, C.c_uint64,
+                                        C.c_uint64, C.POINTER(D64P), D64P, C.c_int]
+        L.orc_mttkrp_f64_mt.restype = C.c_int
+        L.orc_cp_resid_f64_mt.argtypes = [C.c_int, C.c_uint64, U32PP, F32P, C.c_uint64, C.POINTER(D64P),
+                                          D64P, D64P, C.c_int]
+        L.orc_cp_resid_f64_mt.restype = C.c_int
         _LIB = L
     return _LIB
 
@@ -231,6 +239,57 @@ def mttkrp_coo_f64(inds, vals, mode, U, dims=None):
     lib().orc_mttkrp_coo_f64(len(inds), len(vals), _parr(inds, U64P), vals.ctypes.data_as(F32P),
                              int(mode), R, _parr(U, F32P), out.ctypes.data_as(C.POINTER(C.c_double)))
     return out
+
+
+def default_threads():
+    """Host threads for the split oracle: at most 16 (the CPU share of one GPU)."""
+    return max(1, min(16, os.cpu_count() or 1))
+
+
+def _u32(a):
+    return np.ascontiguousarray(a, dtype=np.uint32)
+
+
+def mttkrp_f64_mt(inds, vals, mode, U, rows=None, threads=None):
+    """Float64 MTTKRP split over host threads by output-row ranges; equal bit
+    for bit to ``mttkrp_coo_f64`` (each row summed in stored order).  ``vals``
+    may be float32 (tensor values) or float64 (e.g. residuals); ``U`` are
+    converted to float64 copies."""
+    inds = [_u32(a) for a in inds]
+    nnz = len(inds[0])
+    U = [np.ascontiguousarray(u, dtype=np.float64) for u in U]
+    R = U[0].shape[1]
+    rows = U[mode].shape[0] if rows is None else rows
+    out = np.empty((rows, R), dtype=np.float64)
+    D64P = C.POINTER(C.c_double)
+    F32P = C.POINTER(C.c_float)
+    vals = np.ascontiguousarray(vals)
+    vf = vals.ctypes.data_as(F32P) if vals.dtype == np.float32 else None
+    if vals.dtype != np.float32:
+        vals = np.ascontiguousarray(vals, dtype=np.float64)
+    vd = vals.ctypes.data_as(D64P) if vals.dtype == np.float64 else None
+    rc = lib().orc_mttkrp_f64_mt(len(inds), nnz, _parr(inds, C.POINTER(C.c_uint32)), vf, vd, int(mode), rows, R,
+                                 _parr(U, D64P), out.ctypes.data_as(D64P), int(threads or default_threads()))
+    if rc != 0:
+        raise RuntimeError("orc_mttkrp_f64_mt failed")
+    return out
+
+
+def cp_resid_f64_mt(inds, vals, U, threads=None):
+    """(1/2 sum e^2, e) with e_x = sum_r prod_n U_n[i_n(x)][r] - v_x in float64."""
+    inds = [_u32(a) for a in inds]
+    nnz = len(inds[0])
+    U = [np.ascontiguousarray(u, dtype=np.float64) for u in U]
+    vals = np.ascontiguousarray(vals, dtype=np.float32)
+    e = np.empty(nnz, dtype=np.float64)
+    loss = C.c_double(0.0)
+    D64P = C.POINTER(C.c_double)
+    rc = lib().orc_cp_resid_f64_mt(len(inds), nnz, _parr(inds, C.POINTER(C.c_uint32)),
+                                   vals.ctypes.data_as(C.POINTER(C.c_float)), U[0].shape[1], _parr(U, D64P),
+                                   e.ctypes.data_as(D64P), C.byref(loss), int(threads or default_threads()))
+    if rc != 0:
+        raise RuntimeError("orc_cp_resid_f64_mt failed")
+    return float(loss.value), e
 
 
 # --------------------------------------------------------------------------

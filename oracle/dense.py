@@ -54,8 +54,11 @@ def _solve_normals(G, M):
         return np.linalg.solve(G, M.T).T
 
 
-def cpd_als(inds, vals, dims, U_init, iters, tol=1e-5):
-    """Returns (fits per iteration, final factors, lambda after GetFinalLambda)."""
+def cpd_als(inds, vals, dims, U_init, iters, tol=1e-5, mttkrp=None):
+    """Returns (fits per iteration, final factors, lambda after GetFinalLambda).
+    ``mttkrp(inds, vals, U, mode)`` computes the float64 MTTKRP (default
+    ``mttkrp64``; the threaded C oracle for tensors of 1e7+ nonzeros)."""
+    mttkrp = mttkrp64 if mttkrp is None else mttkrp
     N = len(dims)
     U = [np.array(u, dtype=np.float64) for u in U_init]
     R = U[0].shape[1]
@@ -67,7 +70,7 @@ def cpd_als(inds, vals, dims, U_init, iters, tol=1e-5):
     for it in range(iters):
         Mlast = None
         for m in range(N):
-            M = mttkrp64(inds, vals, U, m)
+            M = mttkrp(inds, vals, U, m)
             if m == N - 1:
                 Mlast = M
             G = np.ones((R, R))

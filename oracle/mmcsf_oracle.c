@@ -29,6 +29,11 @@
  *                          m+0.5), brute-force float64 einsum on tiny random
  *                          tensors, rank-1 closed form, the cross-mode
  *                          inner-product identity.
+ *   orc_mttkrp_f64_mt      identical, bit for bit, to orc_mttkrp_coo_f64 for
+ *                          every thread count (every row is summed in stored
+ *                          order), on 3-D and 4-D skewed tensors; toy.tns.
+ *   orc_cp_resid_f64_mt    numpy float64 model values (dense.cp_loss) and a
+ *                          planted exact model with zero residual.
  */
 #include <stdint.h>
 #include <stdlib.h>
@@ -452,4 +457,149 @@ void orc_mttkrp_coo_f64(int ndims, uint64_t nnz, const uint64_t *const *inds,
             o[r] += t;
         }
     }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Float64 MTTKRP split over host threads, for tensors of 1e7-1e8 nonzeros. */
+/* Same contraction and role map as orc_mttkrp_coo_f64 (mttkrp_cpu.cpp:     */
+/* 60-113 in float64): thread t owns a contiguous range of output rows with */
+/* about nnz/T nonzeros and scans the nonzeros in stored order, keeping     */
+/* those of its rows, so every row is summed in the same order as the       */
+/* sequential function and the result is identical to it bit for bit.       */
+/* Coordinates are uint32 (the device layout); values are float (vf) or     */
+/* double (vd, e.g. residuals); factors are double row-major dims[m] x R.   */
+/* ------------------------------------------------------------------------ */
+#include <pthread.h>
+
+typedef struct {
+    int ndims, mode;
+    uint64_t nnz, R, lo, hi;
+    const uint32_t *const *inds;
+    const float *vf;
+    const double *vd;
+    const double *const *U;
+    double *out;
+} orc_mt_job;
+
+static void *orc_mttkrp_f64_rows(void *arg)
+{
+    const orc_mt_job *j = (const orc_mt_job *)arg;
+    int cm[ORC_MAXD];
+    for (int m = 0; m < j->ndims; ++m) cm[m] = (m + j->mode) % j->ndims;
+    const uint32_t *row = j->inds[cm[0]];
+    const uint64_t R = j->R;
+    for (uint64_t x = 0; x < j->nnz; ++x) {
+        const uint64_t i = row[x];
+        if (i < j->lo || i >= j->hi) continue;
+        const double *a = j->U[cm[1]] + (uint64_t)j->inds[cm[1]][x] * R;
+        const double *b = j->U[cm[2]] + (uint64_t)j->inds[cm[2]][x] * R;
+        const double *c = j->ndims == 4 ? j->U[cm[3]] + (uint64_t)j->inds[cm[3]][x] * R : 0;
+        double *o = j->out + i * R;
+        const double v = j->vf ? (double)j->vf[x] : j->vd[x];
+        for (uint64_t r = 0; r < R; ++r) {
+            double t = v * a[r] * b[r];
+            if (c) t *= c[r];
+            o[r] += t;
+        }
+    }
+    return 0;
+}
+
+/* out (dims[mode] x R) is overwritten.  Returns 0, or -1 if a thread could */
+/* not be started or memory ran out.                                        */
+int orc_mttkrp_f64_mt(int ndims, uint64_t nnz, const uint32_t *const *inds,
+                      const float *vf, const double *vd, int mode,
+                      uint64_t rows, uint64_t R, const double *const *U,
+                      double *out, int nthreads)
+{
+    if (nthreads < 1) nthreads = 1;
+    if (nthreads > 64) nthreads = 64;
+    memset(out, 0, rows * R * sizeof(double));
+    const uint32_t *row = inds[(0 + mode) % ndims];
+    uint64_t *cnt = (uint64_t *)calloc(rows + 1, sizeof(uint64_t));
+    if (!cnt) return -1;
+    for (uint64_t x = 0; x < nnz; ++x) cnt[row[x]]++;
+    /* split points: thread t takes rows up to the first prefix >= (t+1)/T */
+    uint64_t bound[65];
+    bound[0] = 0;
+    int t = 0;
+    uint64_t acc = 0;
+    for (uint64_t i = 0; i < rows && t < nthreads - 1; ++i) {
+        acc += cnt[i];
+        while (t < nthreads - 1 && acc * (uint64_t)nthreads >= (uint64_t)(t + 1) * nnz) bound[++t] = i + 1;
+    }
+    while (t < nthreads - 1) { bound[t + 1] = bound[t]; ++t; }
+    bound[nthreads] = rows;
+    free(cnt);
+    orc_mt_job job[64];
+    pthread_t th[64];
+    int started[64] = {0};
+    int rc = 0;
+    for (int k = 0; k < nthreads; ++k) {
+        job[k] = (orc_mt_job){ndims, mode, nnz, R, bound[k], bound[k + 1], inds, vf, vd, U, out};
+        if (bound[k] == bound[k + 1]) continue;
+        if (pthread_create(&th[k], 0, orc_mttkrp_f64_rows, &job[k]) != 0) { rc = -1; break; }
+        started[k] = 1;
+    }
+    for (int k = 0; k < nthreads; ++k)
+        if (started[k]) pthread_join(th[k], 0);
+    return rc;
+}
+
+/* ------------------------------------------------------------------------ */
+/* CP-completion residuals in float64 over host threads (no reference       */
+/* counterpart; the loss is the least-squares CP loss over the observed     */
+/* entries):  e_x = sum_r prod_n U_n[i_n(x)][r] - v_x,  returns 1/2 sum e^2 */
+/* summed per thread over contiguous nonzero ranges, then in thread order.  */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+    int ndims;
+    uint64_t lo, hi, R;
+    const uint32_t *const *inds;
+    const float *v;
+    const double *const *U;
+    double *e;
+    double sq;
+} orc_res_job;
+
+static void *orc_resid_range(void *arg)
+{
+    orc_res_job *j = (orc_res_job *)arg;
+    double sq = 0.0;
+    for (uint64_t x = j->lo; x < j->hi; ++x) {
+        double m = 0.0;
+        for (uint64_t r = 0; r < j->R; ++r) {
+            double p = 1.0;
+            for (int n = 0; n < j->ndims; ++n) p *= j->U[n][(uint64_t)j->inds[n][x] * j->R + r];
+            m += p;
+        }
+        const double e = m - (double)j->v[x];
+        if (j->e) j->e[x] = e;
+        sq += e * e;
+    }
+    j->sq = sq;
+    return 0;
+}
+
+int orc_cp_resid_f64_mt(int ndims, uint64_t nnz, const uint32_t *const *inds,
+                        const float *vals, uint64_t R, const double *const *U,
+                        double *e, double *loss, int nthreads)
+{
+    if (nthreads < 1) nthreads = 1;
+    if (nthreads > 64) nthreads = 64;
+    orc_res_job job[64];
+    pthread_t th[64];
+    int started[64] = {0};
+    int rc = 0;
+    for (int k = 0; k < nthreads; ++k) {
+        job[k] = (orc_res_job){ndims, nnz * (uint64_t)k / (uint64_t)nthreads,
+                               nnz * (uint64_t)(k + 1) / (uint64_t)nthreads, R, inds, vals, U, e, 0.0};
+        if (pthread_create(&th[k], 0, orc_resid_range, &job[k]) != 0) { rc = -1; break; }
+        started[k] = 1;
+    }
+    double sq = 0.0;
+    for (int k = 0; k < nthreads; ++k)
+        if (started[k]) { pthread_join(th[k], 0); sq += job[k].sq; }
+    *loss = 0.5 * sq;
+    return rc;
 }

@@ -539,3 +539,57 @@ def test_ttm_then_upper_levels_give_root_mttkrp(dims):
                         out[idx[0][s]] += Y[f] * U[order[2]][idx[2][f]] * U[order[1]][idx[1][j]]
         ref = O.mttkrp_coo_f64(inds, vals, order[0], [u.astype(np.float32) for u in U])
         np.testing.assert_allclose(out, ref, rtol=1e-5, atol=1e-5)
+
+
+# ---------------------------------------------------------------------------
+# threaded float64 oracle (used at the full benchmark shapes)
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("dims,skew", [([300, 200, 100], [1.3, 1.2, None]), ([50, 40, 30, 20], [1.2, None, 1.1, None])])
+def test_threaded_mttkrp_equals_sequential_bit_for_bit(dims, skew):
+    rng = np.random.default_rng(len(dims))
+    inds, vals = random_tensor(rng, dims, 60000, skew)
+    U = [rng.standard_normal((d, 12)).astype(np.float32) for d in dims]
+    for mode in range(len(dims)):
+        seq = O.mttkrp_coo_f64(inds, vals, mode, U)
+        for T in (1, 2, 5, 16, 64):
+            np.testing.assert_array_equal(O.mttkrp_f64_mt(inds, vals, mode, U, threads=T), seq)
+    # float64 values (residuals) take the same path: a float64 copy of the
+    # float32 values gives the same result
+    np.testing.assert_array_equal(O.mttkrp_f64_mt(inds, vals.astype(np.float64), 0, U),
+                                  O.mttkrp_coo_f64(inds, vals, 0, U))
+
+
+def test_threaded_mttkrp_toy_closed_form(golden_dir):
+    dims, inds, vals = O.load_tns(f"{golden_dir}/toy.tns")
+    U = [np.full((d, 3), m + 0.5, dtype=np.float32) for m, d in enumerate(dims)]
+    for mode, rows in TOY_EXPECT.items():
+        np.testing.assert_array_equal(O.mttkrp_f64_mt(inds, vals, mode, U, threads=4),
+                                      np.repeat(np.array(rows)[:, None], 3, 1))
+
+
+def test_threaded_mttkrp_empty_and_single_row():
+    U = [np.ones((d, 4), np.float32) for d in (5, 6, 7)]
+    empty = [np.zeros(0, np.int64)] * 3
+    assert not O.mttkrp_f64_mt(empty, np.zeros(0, np.float32), 0, U, threads=8).any()
+    inds = [np.full(1000, 3), np.arange(1000) % 6, np.arange(1000) % 7]
+    out = O.mttkrp_f64_mt(inds, np.ones(1000, np.float32), 0, U, threads=8)
+    assert out[3, 0] == 1000.0 and out.sum() == 4000.0
+
+
+def test_threaded_cp_residuals_match_numpy():
+    from oracle import dense as D
+    rng = np.random.default_rng(9)
+    for dims in ([40, 30, 20], [12, 9, 8, 7]):
+        inds, vals = random_tensor(rng, dims, 3000)
+        U = [rng.standard_normal((d, 6)) for d in dims]
+        L, e = O.cp_resid_f64_mt(inds, vals, U, threads=7)
+        L64, e64 = D.cp_loss(inds, vals, U)
+        np.testing.assert_allclose(e, -e64, rtol=0, atol=1e-12)
+        assert abs(L - L64) <= 1e-12 * L64
+    # an exact planted model has zero residual
+    dims = [10, 9, 8]
+    inds, _ = random_tensor(rng, dims, 300)
+    A = [rng.integers(-4, 5, (d, 3)) / 4.0 for d in dims]
+    v = D.cp_model_values(inds, A).astype(np.float32)   # exact in float32 (dyadic)
+    L, e = O.cp_resid_f64_mt(inds, v, A)
+    assert L == 0.0 and not e.any()
