@@ -32,6 +32,13 @@ for _p in (ROOT, os.path.join(ROOT, "mm-csf_amd")):
 METRIC = "All-mode MTTKRP ms & HBM GB/s, FP32 vs BF16-factor at <=1e-3 rel err"
 HBM_PEAK_GBS = 8000.0
 SEED = 0x4D4D43534600 + 1
+# Random factor-row gather ceiling of one MI355X: 128-byte rows (float32,
+# R=32) gathered by groups of 8 lanes x 16 B, uniformly at random from a
+# 6.4 MB table (the leaf table of the mode-0 and mode-1 trees of this tensor),
+# by a kernel that does nothing else: 162 G rows/s (profiles/gather_ceiling.md).
+GATHER_CEILING_G_ROWS_F32 = 162.0
+GATHER_NOTE = ("random 128-byte row gathers per second of a gather-only kernel on a 6.4 MB table "
+               "(profiles/gather_ceiling.md)")
 
 
 def log(msg):
@@ -87,43 +94,76 @@ def rel_err(out, ref):
     return float(torch.linalg.norm(d) / torch.linalg.norm(ref)), float(d.abs().max() / ref.abs().max())
 
 
-def cpu_baseline(inds, vals, U, dims, sample, bytes_f32):
-    """The reference's -t 1 path (sequential float32 COO MTTKRP) restated in
-    C (oracle/), one core, on the first `sample` nonzeros sorted like
-    sort_COOtensor; time scaled to the full nonzero count."""
+def cpu_model():
+    """CPU model name of this host (from /proc/cpuinfo)."""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def sorted_for_t1(torch, inds, vals, dims, mode):
+    """The nonzeros ordered as sort_COOtensor leaves them before -t 1
+    (mttkrp.cu:79): by modeOrder[0], then modeOrder[1]; sorted on the GPU,
+    returned on the host."""
+    import numpy as np
+
+    import oracle as O
+    order = O.order_tensormode(dims, mode)
+    key = inds[order[0]].long() * dims[order[1]] + inds[order[1]].long()
+    perm = torch.sort(key, stable=True).indices
+    hi = [a[perm].cpu().numpy().astype(np.int64) for a in inds]
+    return hi, vals[perm].cpu().numpy()
+
+
+def cpu_baseline(torch, inds, vals, U, dims, bytes_f32):
+    """The reference's -t 1 path (sequential float32 COO MTTKRP,
+    mttkrp_cpu.cpp:60-113) restated in C (oracle/), one core, on every
+    nonzero of the tensor, each mode timed over its loop only
+    (mttkrp.cu:106-108) with the nonzeros sorted like sort_COOtensor."""
     import numpy as np
 
     import oracle as O
 
     nnz = vals.numel()
-    S = min(sample, nnz)
-    hi = [a[:S].cpu().numpy().astype(np.int64) for a in inds]
-    hv = vals[:S].cpu().numpy()
     hU = [u.cpu().numpy() for u in U]
     per_mode = []
     for n in range(len(dims)):
-        order = O.order_tensormode(dims, n)
-        perm = np.lexsort((hi[order[1]], hi[order[0]]))   # first two keys (util.h:389-397)
-        si = [a[perm] for a in hi]
+        si, sv = sorted_for_t1(torch, inds, vals, dims, n)
         out = np.zeros((dims[n], hU[0].shape[1]), dtype=np.float32)
         t0 = time.perf_counter()
-        O.mttkrp_coo_f32(si, hv[perm], n, hU, out)
+        O.mttkrp_coo_f32(si, sv, n, hU, out)
         per_mode.append(time.perf_counter() - t0)
-        log(f"cpu baseline mode {n}: {per_mode[-1]:.2f} s on {S} nonzeros")
-    scale = nnz / S
-    full_s = [t * scale for t in per_mode]
+        log(f"cpu baseline mode {n}: {per_mode[-1]:.2f} s on {nnz} nonzeros")
+        del si, sv, out
     return {
-        "value": round(sum(bytes_f32) / sum(full_s) / 1e9, 3),
+        "value": round(sum(bytes_f32) / sum(per_mode) / 1e9, 3),
         "unit": "GB/s",
         "cores": 1,
         "kind": "port",
-        "sample": (f"first {S} of {nnz} nonzeros per mode, sorted by modeOrder[0..1]; "
-                   f"single-thread float32 COO MTTKRP (mttkrp_cpu.cpp:60-113 restated), "
-                   f"time x{scale:.2f} to the full tensor"),
-        "all_mode_ms_extrapolated": round(sum(full_s) * 1e3, 1),
-        "sample_seconds": [round(t, 3) for t in per_mode],
+        "sample": (f"all {nnz} nonzeros, every mode, sorted by modeOrder[0..1]; single-thread float32 "
+                   f"COO MTTKRP (mttkrp_cpu.cpp:60-113 restated in C, gcc -O2), loop only"),
+        "all_mode_ms": round(sum(per_mode) * 1e3, 1),
+        "mode_seconds": [round(t, 3) for t in per_mode],
+        "cpu_model": cpu_model(),
         "host_cores_available": os.cpu_count(),
     }
+
+
+def gathered_rows(info, single_tree=False):
+    """Factor rows a fiber-level-factored traversal must gather per mode: one
+    leaf row per nonzero and one row per entry of every level between the
+    root and the leaf (the fiber-level rows, each read once per fiber)."""
+    N = info["ndims"]
+    out = []
+    for n in range(N):
+        lv = info["nLevel"] if single_tree else info["modeLevel"][n]
+        out.append(info["totNnz"] + sum(lv[l] for l in range(1, N - 1)))
+    return out
 
 
 def run_c2(args):
@@ -146,7 +186,6 @@ def run_c2(args):
     build_s = time.perf_counter() - t0
     info = fmt.info()
     log(f"built in {build_s:.2f} s: {info}")
-    log(f"per-mode tree levels: {info.get('modeLevel')}")
     inds, vals = X.to_torch(dev)
     distinct = [int((torch.bincount(inds[m].long(), minlength=dims[m]) > 0).sum()) for m in range(3)]
     g = torch.Generator(device=dev).manual_seed(SEED)
@@ -155,6 +194,7 @@ def run_c2(args):
     outs = [torch.empty((d, R), dtype=torch.float32, device=dev) for d in dims]
     bytes_f32 = algorithmic_bytes(info, distinct, R, 4)
     bytes_bf16 = algorithmic_bytes(info, distinct, R, 2)
+    rows = gathered_rows(info)
 
     def timed(factors):
         for _ in range(args.warmup):
@@ -168,7 +208,8 @@ def run_c2(args):
         e1.record()
         torch.cuda.synchronize()
         step_ms = e0.elapsed_time(e1) / args.steps
-        # per-kernel durations (HIP events on the launch stream, kernel only)
+        # per-kernel durations (HIP events recorded by the library on the
+        # stream the kernel is launched on, kernel only)
         km = [0.0] * 3
         for _ in range(args.steps):
             _, ms = mmcsf.mttkrp_allmode(fmt, factors, outs, timed=True)
@@ -185,22 +226,15 @@ def run_c2(args):
     log(f"bfloat16: {b16_ms:.3f} ms/step, kernels {b16_k}")
 
     if args.profile:
-        # profiling run: the timed MTTKRP launches above, then a plain 1 GiB
-        # device copy whose counters calibrate HBM byte counts
-        a = torch.ones(1 << 28, dtype=torch.float32, device=dev)
-        b = torch.empty_like(a)
-        for _ in range(3):
-            b.copy_(a)
-        torch.cuda.synchronize()
         print(json.dumps({"metric": METRIC, "profile": True, "ms_per_step": round(f32_ms, 4),
-                          "kernel_ms_f32": f32_k, "kernel_ms_bf16": b16_k, "copy_bytes": 2 * a.numel() * 4}),
-              flush=True)
+                          "kernel_ms_f32": f32_k, "kernel_ms_bf16": b16_k}), flush=True)
         return
 
     # the reference's layout: one MM-CSF tree, non-root modes scatter (LDS
     # privatisation of hot rows, atomics for the rest)
     log("timing the single-tree path")
     fmt1 = mmcsf.Format(X, mmcsf.TYPE_MMCSF, fbr_threshold=128, mode_trees=False)
+    info1 = fmt1.info()
     single_steps = []
     for _ in range(args.warmup):
         mmcsf.mttkrp_allmode(fmt1, U, outs)
@@ -208,10 +242,29 @@ def run_c2(args):
         _, ms = mmcsf.mttkrp_allmode(fmt1, U, outs, timed=True)
         single_steps.append(ms)
     single_k = [sum(s[m] for s in single_steps) / len(single_steps) for m in range(3)]
-    bytes_single = algorithmic_bytes(fmt1.info(), distinct, R, 4, single_tree=True)
+    bytes_single = algorithmic_bytes(info1, distinct, R, 4, single_tree=True)
     del fmt1
     torch.cuda.empty_cache()
     log(f"single tree: kernels {single_k}")
+
+    # B-CSF (-t 8) and HB-CSF (-t 10) rooted at each mode, as the reference's
+    # run_all.sh runs them (one -t 8 / -t 10 run per -m)
+    per_type = {}
+    o1 = [torch.empty((d, R), dtype=torch.float32, device=dev) for d in dims]
+    for typ, name in ((8, "B-CSF"), (10, "HB-CSF")):
+        ms32, ms16 = [], []
+        for m in range(3):
+            f = mmcsf.Format(X, typ, mode=m, fbr_threshold=128)
+            ms32.append(_event_ms(torch, lambda: mmcsf.mttkrp(f, m, U, o1[m]), args.steps, args.warmup))
+            ms16.append(_event_ms(torch, lambda: mmcsf.mttkrp(f, m, Ub, o1[m]), args.steps, args.warmup))
+            del f
+            torch.cuda.empty_cache()
+        per_type[name] = {"type": typ, "f32_ms_per_mode": [round(v, 4) for v in ms32],
+                          "f32_all_mode_ms": round(sum(ms32), 4),
+                          "bf16_ms_per_mode": [round(v, 4) for v in ms16],
+                          "bf16_all_mode_ms": round(sum(ms16), 4)}
+        log(f"-t {typ} per mode: f32 {ms32}, bf16 {ms16}")
+    del o1
 
     log("float64 reference")
     err32, err16 = [], []
@@ -226,12 +279,7 @@ def run_c2(args):
     dom = max(range(3), key=lambda m: f32_k[m])
     achieved = bytes_f32[dom] / (f32_k[dom] * 1e-3) / 1e9
     flops = 3 * args.nnz * R * 3
-    traffic = None
-    tfile = os.path.join(ROOT, "profiles", "k_walk_c2_traffic.json")
-    if os.path.exists(tfile):
-        with open(tfile) as fh:
-            tr = json.load(fh)
-        traffic = tr.get("modes", {}).get(str(dom), {}).get("hbm_bytes")
+    gbytes = lambda b: round(b / 1e9, 3)  # noqa: E731
 
     result = {
         "metric": METRIC,
@@ -247,21 +295,28 @@ def run_c2(args):
         "dtype": "f32",
         "data": "synthetic (GPU generator, seed 0x4D4D43534601)",
         "config": {
-            "workload": ("all-mode MM-CSF MTTKRP (-t 12 -f 128), 500000x200000x50000, "
-                         f"{args.nnz} distinct nonzeros, Zipf1.2 x Zipf1.2 x uniform, R=32"),
-            "R": R, "nnz": args.nnz, "dims": dims, "fbr_threshold": 128,
+            "workload": ("all-mode MM-CSF MTTKRP, -t 12 -f 128 -a 1 (the reference's MM-CSF tree plus a root "
+                         "tree per other mode, so every mode is a root traversal: "
+                         f"{gbytes(info['deviceBytes'])} GB of structure; the reference's single tree, -a 0, "
+                         f"holds {gbytes(info1['deviceBytes'])} GB and is timed under single_tree), "
+                         f"500000x200000x50000, {args.nnz} distinct nonzeros, Zipf1.2 x Zipf1.2 x uniform, R=32; "
+                         "-w 1 and -w 4 give the same launch (the walk splits work by chunk cost, not per slice)"),
+            "R": R, "nnz": args.nnz, "dims": dims, "fbr_threshold": 128, "layout": "-a 1",
+            "structure_bytes": {"-a 1": info["deviceBytes"], "-a 0": info1["deviceBytes"]},
             "mode_order": info["modeOrder"], "n_slices": info["nLevel"][0],
-            "n_fibers": info["nLevel"][1], "distinct_rows": distinct,
+            "n_fibers": info["nLevel"][1], "mode_tree_levels": info["modeLevel"], "distinct_rows": distinct,
         },
         "gflops": round(flops / (f32_ms * 1e-3) / 1e9, 1),
         "modes_f32": [{"mode": m, "kernel_ms": round(f32_k[m], 4), "alg_bytes": bytes_f32[m],
-                       "gbs": round(bytes_f32[m] / (f32_k[m] * 1e-3) / 1e9, 1),
+                       "gbs": round(bytes_f32[m] / (f32_k[m] * 1e-3) / 1e9, 1), "gathered_rows": rows[m],
+                       "g_rows_per_s": round(rows[m] / (f32_k[m] * 1e-3) / 1e9, 1),
                        "rel_err_fp64": err32[m][0], "max_rel_err_fp64": err32[m][1]} for m in range(3)],
         "bf16": {
             "ms_per_step": round(b16_ms, 4),
             "gbs": round(sum(bytes_bf16) / (b16_ms * 1e-3) / 1e9, 2),
             "speedup_vs_f32": round(f32_ms / b16_ms, 3),
             "modes": [{"mode": m, "kernel_ms": round(b16_k[m], 4), "alg_bytes": bytes_bf16[m],
+                       "g_rows_per_s": round(rows[m] / (b16_k[m] * 1e-3) / 1e9, 1),
                        "rel_err_fp64": err16[m][0], "max_rel_err_fp64": err16[m][1]} for m in range(3)],
         },
         "roofline": {
@@ -271,20 +326,32 @@ def run_c2(args):
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": traffic,
+            # HBM bytes per launch are not measurable with the L2 counters
+            # (FETCH_SIZE counts L2-to-fabric requests, most of which the
+            # 256 MB MALL serves); the per-launch L2-fabric request bytes are
+            # in the committed rocprofv3 summary of this kernel
+            "traffic": None,
+            "gather": {
+                "rows_per_launch": rows[dom],
+                "achieved_g_rows_per_s": round(rows[dom] / (f32_k[dom] * 1e-3) / 1e9, 1),
+                "ceiling_g_rows_per_s": GATHER_CEILING_G_ROWS_F32,
+                "frac": round(rows[dom] / (f32_k[dom] * 1e-3) / 1e9 / GATHER_CEILING_G_ROWS_F32, 4),
+                "note": GATHER_NOTE,
+            },
         },
         "single_tree": {
-            "note": "the reference's MM-CSF layout: one tree, modes other than its root scatter",
+            "note": "the reference's MM-CSF layout (-a 0): one tree, modes other than its root scatter",
             "kernel_ms": [round(k, 4) for k in single_k],
             "all_mode_kernel_ms": round(sum(single_k), 4),
             "gbs": round(sum(bytes_single) / (sum(single_k) * 1e-3) / 1e9, 2),
         },
+        "per_mode_types": per_type,
         "generate_s": round(gen_s, 2),
         "build_s": round(build_s, 2),
     }
     if not args.no_cpu:
         log("cpu baseline")
-        result["cpu_baseline"] = cpu_baseline(inds, vals, U, dims, args.cpu_sample, bytes_f32)
+        result["cpu_baseline"] = cpu_baseline(torch, inds, vals, U, dims, bytes_f32)
     print(json.dumps(result), flush=True)
 
 
@@ -689,10 +756,9 @@ def main():
     ap.add_argument("--nnz-c4", type=int, default=20_000_000)
     ap.add_argument("--nnz-c5", type=int, default=50_000_000)
     ap.add_argument("--nnz-next", type=int, default=20_000_000)
-    ap.add_argument("--cpu-sample", type=int, default=50_000_000)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--profile", action="store_true",
-                    help="c2: only the timed MTTKRP launches and a 1 GiB copy (for rocprofv3 runs)")
+                    help="c2: only the timed MTTKRP launches (for rocprofv3 runs)")
     ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4", "c5", "next"],
                     help="c2 (default): the headline all-mode MTTKRP line; c1/c3/c4/c5: the other "
                          "configurations; next: the .tns reader, one-tree types and TTM; one JSON line each")

@@ -187,6 +187,9 @@ typedef struct mmcsf_format_info {
     int modeTrees;             /* internal root trees held besides the MM-CSF tree */
     uint64_t modeLevel[4][4];  /* MM-CSF: entries per level of the tree the MTTKRP of
                                   mode m traverses (its root tree, else the MM-CSF tree) */
+    uint64_t deviceBytes;      /* device memory held by the format (every tree, its
+                                  pointer / index / value arrays, start bitmasks, chunk
+                                  costs, bins and hot-row tables) */
 } mmcsf_format_info;
 
 int mmcsf_format_info_get(const mmcsf_format *f, mmcsf_format_info *info);

@@ -254,6 +254,30 @@ int redundant_mode(int ndims, const uint64_t *dims) {
 
 using namespace mmcsf;
 
+namespace {
+template <typename T>
+uint64_t bytes_of(const mmcsf::DevArray<T> &a) { return a.n * sizeof(T); }
+
+uint64_t csf_bytes(const mmcsf::Csf &c) {
+    uint64_t b = bytes_of(c.leaf) + bytes_of(c.vals) + bytes_of(c.pos) + bytes_of(c.fbrSlc) + bytes_of(c.fbit) +
+                 bytes_of(c.sbit) + bytes_of(c.chunkCost) + bytes_of(c.binList);
+    for (int l = 0; l < mmcsf::kMaxDims - 1; ++l) b += bytes_of(c.ptr[l]) + bytes_of(c.idx[l]);
+    for (int l = 0; l < mmcsf::kMaxDims; ++l) b += bytes_of(c.hot[l].rows) + bytes_of(c.hot[l].tgt);
+    return b;
+}
+
+uint64_t format_bytes(const mmcsf::Format &f) {
+    uint64_t b = csf_bytes(f.csf) + bytes_of(f.cooVals) + csf_bytes(f.hyb.csf) + bytes_of(f.hyb.cooVal) +
+                 bytes_of(f.hyb.cslPtr) + bytes_of(f.hyb.cslIdx) + bytes_of(f.hyb.cslInd1) +
+                 bytes_of(f.hyb.cslInd2) + bytes_of(f.hyb.cslVal) + bytes_of(f.hyb.cslBinList);
+    for (int l = 0; l < mmcsf::kMaxDims; ++l) {
+        b += bytes_of(f.cooKeys[l]) + bytes_of(f.hyb.cooInd[l]);
+        if (f.hasTree[l]) b += csf_bytes(f.tree[l]);
+    }
+    return b;
+}
+}  // namespace
+
 extern "C" {
 
 const char *mmcsf_last_error(void) { return g_last_error.c_str(); }
@@ -578,8 +602,12 @@ int mmcsf_format_build(const mmcsf_tensor *th, int type, const mmcsf_options *op
             if (opt->modeTrees) {
                 // a root tree for every other mode: each MTTKRP becomes a root
                 // traversal.  These trees are internal, so their level order is
-                // free: 3-D trees take whichever of the two orders below the
-                // root has fewer level-1 entries (fewer fiber-level gathers).
+                // free.  The leaf level is gathered once per nonzero, the
+                // level-1 rows once per fiber segment: a 3-D tree puts at its
+                // leaf the mode whose used rows concentrate in a clearly
+                // smaller working set (rows holding 90% of the nonzeros; those
+                // stay in the L2), and otherwise takes the order with fewer
+                // level-1 entries (fewer fiber-level gathers).
                 auto build_tree = [&](const int *o2, Csf &c) {
                     DevArray<uint32_t> k2[kMaxDims];
                     DevArray<float> v2;
@@ -588,16 +616,27 @@ int mmcsf_format_build(const mmcsf_tensor *th, int type, const mmcsf_options *op
                     build_csf(t.ndims, t.dims, t.nnz, o2, k2, std::move(v2), opt->fbrThreashold, kNoSplit, c, s);
                     c.pos = std::move(p2);
                 };
+                uint64_t hot[kMaxDims] = {0, 0, 0, 0};
+                if (t.ndims == 3)
+                    for (int m = 0; m < t.ndims; ++m) hot[m] = hot_footprint(t, m, 0.9, s);
                 for (int m = 0; m < t.ndims; ++m) {
                     if (m == order[0]) continue;
                     int o2[kMaxDims];
                     order_tensormode(t.ndims, t.dims, m, nat, o2);
-                    build_tree(o2, f.tree[m]);
                     if (t.ndims == 3) {
                         int o3[kMaxDims] = {o2[0], o2[2], o2[1], 0};
-                        Csf alt;
-                        build_tree(o3, alt);
-                        if (alt.cnt[1] < f.tree[m].cnt[1]) f.tree[m] = std::move(alt);
+                        if (2 * hot[o3[2]] < hot[o2[2]]) {
+                            build_tree(o3, f.tree[m]);
+                        } else if (2 * hot[o2[2]] < hot[o3[2]]) {
+                            build_tree(o2, f.tree[m]);
+                        } else {
+                            build_tree(o2, f.tree[m]);
+                            Csf alt;
+                            build_tree(o3, alt);
+                            if (alt.cnt[1] < f.tree[m].cnt[1]) f.tree[m] = std::move(alt);
+                        }
+                    } else {
+                        build_tree(o2, f.tree[m]);
                     }
                     f.hasTree[m] = true;
                 }
@@ -669,6 +708,7 @@ int mmcsf_format_info_get(const mmcsf_format *fh, mmcsf_format_info *info) {
                 const Csf &t = f.hasTree[m] ? f.tree[m] : f.csf;
                 for (int l = 0; l < f.ndims; ++l) info->modeLevel[m][l] = t.cnt[l];
             }
+        info->deviceBytes = format_bytes(f);
     });
 }
 

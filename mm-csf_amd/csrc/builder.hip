@@ -216,13 +216,16 @@ __global__ void k_chunk_starts(int N, const uint32_t *__restrict__ p0, const uin
     atomicAdd(cnt + (x >> 6), 1u);
 }
 
-// traversal cost of a chunk: 64 units, wd more when several slices start in
-// it (group-run path and the scan that joins the runs), ws per slice start
-// (row writes); the weights (96, 24) come from timing Zipf-rooted trees
-__global__ void k_chunk_cost(const uint32_t *__restrict__ cnt, uint64_t n, bool plain, uint32_t wd, uint32_t ws,
-                             uint32_t *cost) {
+// traversal cost of a chunk: 64 units (its leaf rows), wf per leaf fiber
+// starting in it (the fiber-level rows, read once per fiber segment) and ws
+// per slice start (row writes and the join of the group runs)
+__global__ void k_chunk_cost(const uint32_t *__restrict__ cnt, const uint32_t *__restrict__ fbit, uint64_t n,
+                             bool plain, uint32_t wf, uint32_t ws, uint32_t *cost) {
     uint64_t c = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
-    if (c < n) cost[c] = plain ? 1u : 64u + (cnt[c] >= 3 ? wd : 0u) + ws * cnt[c];
+    if (c < n) {
+        const uint32_t fs = (uint32_t)(__popc(fbit[2 * c]) + __popc(fbit[2 * c + 1]));
+        cost[c] = plain ? 1u : 64u + wf * fs + ws * cnt[c];
+    }
 }
 
 // bit ptr[i] of `bits` for every entry i < n (first child of every entry)
@@ -570,12 +573,13 @@ void build_walk_bits(Csf &c, hipStream_t s) {
         if (c.cnt[0])
             hipLaunchKernelGGL(k_chunk_starts, dim3(nblk(c.cnt[0])), dim3(kBS), 0, s, N, c.ptr[0].p, c.ptr[1].p,
                                N == 4 ? c.ptr[2].p : (const uint32_t *)nullptr, c.cnt[0], cnt.p);
-        // the weighted total (at most chunks * (64 + wd) + ws * slices, every
+        // the weighted total (at most chunks * 64 (1 + wf) + ws * slices, every
         // slice starting in one chunk) must fit the 32-bit prefix; else split
         // by chunks
-        const uint32_t wd = 96, ws = 24;
-        const bool plain = chunks * (64ull + wd) + ws * (uint64_t)c.cnt[0] >= 0xF0000000ull;
-        hipLaunchKernelGGL(k_chunk_cost, dim3(nblk(chunks)), dim3(kBS), 0, s, cnt.p, chunks, plain, wd, ws, cost.p);
+        const uint32_t wf = 1, ws = 24;
+        const bool plain = chunks * 64ull * (1 + wf) + ws * (uint64_t)c.cnt[0] >= 0xF0000000ull;
+        hipLaunchKernelGGL(k_chunk_cost, dim3(nblk(chunks)), dim3(kBS), 0, s, cnt.p, c.fbit.p, chunks, plain, wf, ws,
+                           cost.p);
         uint64_t total = exclusive_sum(cost.p, c.chunkCost.p, chunks, s, tmp);
         hipLaunchKernelGGL(k_set, dim3(1), dim3(1), 0, s, c.chunkCost.p, chunks, (uint32_t)total);
     } else {
@@ -939,6 +943,27 @@ void build_hot_table(const Csf &c, int level, uint64_t rows, HotTable &h, hipStr
     if (nhot) hipLaunchKernelGGL(k_rank_table, dim3((nhot + 255) / 256), dim3(256), 0, s, ids2.p, nhot, rank.p);
     hipLaunchKernelGGL(k_encode, dim3(nblk(n)), dim3(kBS), 0, s, idx, rank.p, n, h.tgt.p);
     MMCSF_HIP(hipStreamSynchronize(s));
+}
+
+// Rows of tensor mode m that hold `frac` of the nonzeros, the most used rows
+// first: the working set of gathers from that mode's factor table (small for
+// skewed modes, most of the rows for uniform ones).
+uint64_t hot_footprint(const Tensor &t, int m, double frac, hipStream_t s) {
+    const uint64_t rows = t.dims[m], n = t.nnz;
+    if (n == 0 || rows == 0) return 0;
+    DevArray<uint32_t> cnt(rows);
+    MMCSF_HIP(hipMemsetAsync(cnt.p, 0, rows * 4, s));
+    hipLaunchKernelGGL(k_hist, dim3(nblk(n)), dim3(kBS), 0, s, t.inds[m].p, n, cnt.p);
+    MMCSF_HIP(hipGetLastError());
+    std::vector<uint32_t> h(rows);
+    MMCSF_HIP(hipMemcpyAsync(h.data(), cnt.p, rows * 4, hipMemcpyDeviceToHost, s));
+    MMCSF_HIP(hipStreamSynchronize(s));
+    std::sort(h.begin(), h.end(), [](uint32_t a, uint32_t b) { return a > b; });
+    const double want = frac * (double)n;
+    double acc = 0;
+    uint64_t k = 0;
+    while (k < rows && acc < want) acc += h[k++];
+    return k;
 }
 
 }  // namespace mmcsf

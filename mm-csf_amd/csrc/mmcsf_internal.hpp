@@ -180,6 +180,8 @@ void build_csf(int ndims, const uint64_t *dims, uint64_t nnz, const int *order,
 void build_bins_tiled(Csf &c, hipStream_t s);           // make_TiledBin
 void build_hyb(int ndims, const uint64_t *dims, Csf &&src, Hyb &out, hipStream_t s);
 void build_hot_table(const Csf &c, int level, uint64_t rows, HotTable &h, hipStream_t s);
+// rows of tensor mode m holding `frac` of the nonzeros, most used first
+uint64_t hot_footprint(const Tensor &t, int m, double frac, hipStream_t s);
 
 // ---------------------------------------------------------------------------
 // MTTKRP launchers (mttkrp.hip)

@@ -520,9 +520,7 @@ void mttkrp_csf(const Csf &c, const uint64_t *dims, int outLevel, const Factors 
     a.R = F.R;
     a.perGroup = per_group_for(c.nnz, g.G);
     if (outLevel == 0 && walk_supports(c, F.R)) {
-        if (F.dtype == MMCSF_BF16 && F.R % 8 == 0)
-            for (int l = 1; l < N; ++l) check_align(a.U[l], MMCSF_F32, g);  // 16-byte rows
-        mttkrp_root(c, F, out, s);
+        mttkrp_root(c, F, out, s);  // checks the alignment its row loads need
         return;
     }
     if (outLevel > 0) {

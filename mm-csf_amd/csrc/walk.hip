@@ -23,7 +23,11 @@ WalkArgs args_of(const Csf &c, const Factors &F) {
     a.sbit = c.sbit.p;
     a.cost = c.chunkCost.p;
     a.pos = c.pos.p;
-    for (int l = 0; l < N; ++l) a.U[l] = F.U[c.order[l]];
+    const uint64_t es = F.dtype == MMCSF_F32 ? 4 : 2;
+    for (int l = 0; l < N; ++l) {
+        a.U[l] = F.U[c.order[l]];
+        a.tabBytes[l] = (uint32_t)(c.rows[l] * (uint64_t)F.R * es);
+    }
     a.nnz = (uint32_t)c.nnz;
     a.R = F.R;
     a.alpha = 1.f;
@@ -40,12 +44,28 @@ __global__ void k_half(double *p) {
 
 }  // namespace
 
-// the walk addresses factor rows by 32-bit byte offsets: every table of the
-// tree's levels must stay below 4 GiB (counted at 4 bytes per element)
+// the walk addresses factor rows by 32-bit byte offsets and masks the rows it
+// does not need with out-of-range buffer offsets: every table of the tree's
+// levels must stay below 4 GiB - 4 KiB (counted at 4 bytes per element)
 static bool tables_fit(const Csf &c, int R) {
     for (int l = 0; l < c.ndims; ++l)
-        if (c.rows[l] * (uint64_t)R * 4u > 0xFFFFFFFFull) return false;
+        if (c.rows[l] * (uint64_t)R * 4u > 0xFFFFF000ull) return false;
     return true;
+}
+
+// row loads of the walk are VEC elements wide (16 bytes, or 8 bytes for
+// bfloat16 rows with R % 8 != 0); rows of `out` are written as float4
+static void check_walk_align(const Csf &c, const Factors &F, int skipLevel, const float *out) {
+    const uintptr_t need = (F.dtype == MMCSF_BF16 && F.R % 8 != 0) ? 8 : 16;
+    for (int l = 0; l < c.ndims; ++l) {
+        if (l == skipLevel) continue;
+        const void *p = F.U[c.order[l]];
+        MMCSF_REQUIRE(p != nullptr, MMCSF_ERR_ARG, "missing factor matrix");
+        MMCSF_REQUIRE(reinterpret_cast<uintptr_t>(p) % need == 0, MMCSF_ERR_ARG,
+                      "factor pointer is not aligned to " + std::to_string(need) + " bytes");
+    }
+    MMCSF_REQUIRE(!out || reinterpret_cast<uintptr_t>(out) % 16 == 0, MMCSF_ERR_ARG,
+                  "output pointer is not aligned to 16 bytes");
 }
 
 bool walk_supports(const Csf &c, int R) {
@@ -57,6 +77,7 @@ void mttkrp_root(const Csf &c, const Factors &F, float *out, hipStream_t s) {
     if (c.nnz == 0) return;
     MMCSF_REQUIRE(walk_supports(c, F.R), MMCSF_ERR_UNSUPPORTED,
                   "root walk needs R % 4 == 0, start bitmasks and factor tables below 4 GiB");
+    check_walk_align(c, F, 0, out);
     WalkArgs a = args_of(c, F);
     a.out = out;
     walk_launch<kMttkrp>(c, F, a, s);
@@ -97,6 +118,10 @@ void cp_gradient(const Csf &c, const Factors &F, const float *weights, float alp
 void ttm_fibers(const Csf &c, const Factors &F, float *Y, hipStream_t s) {
     if (c.nnz == 0) return;
     MMCSF_REQUIRE(walk_supports(c, F.R), MMCSF_ERR_UNSUPPORTED, "TTM needs R % 4 == 0, R <= 256 and factor tables below 4 GiB");
+    const int leaf = c.ndims - 1;
+    const uintptr_t need = (F.dtype == MMCSF_BF16 && F.R % 8 != 0) ? 8 : 16;
+    MMCSF_REQUIRE(F.U[c.order[leaf]] && reinterpret_cast<uintptr_t>(F.U[c.order[leaf]]) % need == 0, MMCSF_ERR_ARG,
+                  "factor pointer is not aligned to " + std::to_string(need) + " bytes");
     WalkArgs a = args_of(c, F);
     a.out = Y;
     walk_launch<kTtm>(c, F, a, s);

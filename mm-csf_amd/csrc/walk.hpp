@@ -13,10 +13,13 @@
 //     the same way over the fibers; the level-1 entry gives the fiber index
 //     and the slice;
 //   * groups of G lanes gather whole factor rows (16 or 8 bytes per
-//     lane-load).  Chunks with at most two slice starts interleave the
-//     groups (group g takes nonzeros g, g + 64/G, ...); chunks with more give
-//     every group a contiguous run and join the runs by a segmented scan;
-//   * MTTKRP: contribution v * U_leaf[leaf] (.) U_1[idx_1] (. U_2[idx_2]);
+//     lane-load); group g takes the run of nonzeros g*G .. g*G + G-1 of the
+//     chunk, and the runs are joined by a segmented scan over the groups;
+//   * MTTKRP: contribution v * U_leaf[leaf] (.) U_1[idx_1] (. U_2[idx_2]),
+//     factored by fiber: inside a run, v * U_leaf[leaf] is summed per leaf
+//     fiber and multiplied by the fiber's upper-level rows once, which are
+//     read only where a fiber segment of the run begins (out-of-range buffer
+//     loads for the other lanes, so every load stays unconditional);
 //     CP: the model value <U_root[i], U_1[j] (. U_2[k]) (.) U_leaf[l]> is a
 //     dot product reduced over the group, e = model - v, the loss sums e^2
 //     in float64, the gradient contribution is alpha * e * (the MTTKRP
@@ -49,6 +52,7 @@ struct WalkArgs {
     const uint32_t *pos;   // tensor-order position of every stored nonzero
     const float *wts;      // weights in tensor order
     const void *U[4];      // factor of each level
+    uint32_t tabBytes[4];  // bytes of each level's factor table (buffer range)
     float *out;            // root-mode rows (zeroed by the caller)
     float *resid;          // residuals in tensor order, or null
     double *sumsq;         // sum of e^2 (accumulated), or null

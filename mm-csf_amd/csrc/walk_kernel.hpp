@@ -51,8 +51,17 @@ __device__ __forceinline__ uint32_t wave_lower(const uint32_t *__restrict__ a, u
     return lo;
 }
 
+// Buffer resource of one factor table (gfx9 raw buffer: stride 0, byte
+// range checked): lanes whose offset is at or past `bytes` read zeros and
+// access no memory.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t table_rsrc(const void *p, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)bytes, 0x00020000);
+}
+
 // VEC consecutive elements of one row per lane-load (at a byte offset from
-// the table base: tables are below 4 GiB), converted to float
+// the table base: tables are below 4 GiB), converted to float.  load() is a
+// plain global load; loadb() a buffer load whose out-of-range lanes read
+// zeros without touching memory.
 template <typename T, int VEC>
 struct Raw;
 template <>
@@ -61,7 +70,13 @@ struct Raw<float, 4> {
     __device__ __forceinline__ void load(const float *U, uint32_t off) {
         q = *reinterpret_cast<const float4 *>(reinterpret_cast<const char *>(U) + off);
     }
+    __device__ __forceinline__ void loadb(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+        q = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
+    }
     __device__ __forceinline__ void get(float *o) const { o[0] = q.x; o[1] = q.y; o[2] = q.z; o[3] = q.w; }
+    __device__ __forceinline__ void from_lane(const Raw &o, int src) {
+        q.x = __shfl(o.q.x, src, 64); q.y = __shfl(o.q.y, src, 64); q.z = __shfl(o.q.z, src, 64); q.w = __shfl(o.q.w, src, 64);
+    }
 };
 template <>
 struct Raw<uint16_t, 8> {
@@ -69,11 +84,17 @@ struct Raw<uint16_t, 8> {
     __device__ __forceinline__ void load(const uint16_t *U, uint32_t off) {
         q = *reinterpret_cast<const uint4 *>(reinterpret_cast<const char *>(U) + off);
     }
+    __device__ __forceinline__ void loadb(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+        q = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
+    }
     __device__ __forceinline__ void get(float *o) const {
         o[0] = __uint_as_float(q.x << 16); o[1] = __uint_as_float(q.x & 0xffff0000u);
         o[2] = __uint_as_float(q.y << 16); o[3] = __uint_as_float(q.y & 0xffff0000u);
         o[4] = __uint_as_float(q.z << 16); o[5] = __uint_as_float(q.z & 0xffff0000u);
         o[6] = __uint_as_float(q.w << 16); o[7] = __uint_as_float(q.w & 0xffff0000u);
+    }
+    __device__ __forceinline__ void from_lane(const Raw &o, int src) {
+        q.x = __shfl(o.q.x, src, 64); q.y = __shfl(o.q.y, src, 64); q.z = __shfl(o.q.z, src, 64); q.w = __shfl(o.q.w, src, 64);
     }
 };
 template <>
@@ -82,9 +103,15 @@ struct Raw<uint16_t, 4> {
     __device__ __forceinline__ void load(const uint16_t *U, uint32_t off) {
         q = *reinterpret_cast<const uint2 *>(reinterpret_cast<const char *>(U) + off);
     }
+    __device__ __forceinline__ void loadb(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+        q = __builtin_bit_cast(uint2, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+    }
     __device__ __forceinline__ void get(float *o) const {
         o[0] = __uint_as_float(q.x << 16); o[1] = __uint_as_float(q.x & 0xffff0000u);
         o[2] = __uint_as_float(q.y << 16); o[3] = __uint_as_float(q.y & 0xffff0000u);
+    }
+    __device__ __forceinline__ void from_lane(const Raw &o, int src) {
+        q.x = __shfl(o.q.x, src, 64); q.y = __shfl(o.q.y, src, 64);
     }
 };
 
@@ -99,6 +126,40 @@ __device__ __forceinline__ uint64_t window64(const uint32_t *__restrict__ bits, 
     return win;
 }
 
+// value of lane t of this lane's group of G lanes (t a constant after
+// unrolling): ds_swizzle in bit mode, lane' = (lane & ~(G-1)) | t within
+// each half-wave
+template <int G>
+__device__ __forceinline__ uint32_t grp_lane(uint32_t v, int t) {
+    if constexpr (G == 1) {
+        return v;
+    } else {
+        constexpr int am = 0x1F & ~(G - 1);
+        switch (t) {
+            case 0: return (uint32_t)__builtin_amdgcn_ds_swizzle((int)v, am | (0 << 5));
+            case 1: return (uint32_t)__builtin_amdgcn_ds_swizzle((int)v, am | (1 << 5));
+            case 2: return (uint32_t)__builtin_amdgcn_ds_swizzle((int)v, am | (2 << 5));
+            case 3: return (uint32_t)__builtin_amdgcn_ds_swizzle((int)v, am | (3 << 5));
+            case 4: return (uint32_t)__builtin_amdgcn_ds_swizzle((int)v, am | (4 << 5));
+            case 5: return (uint32_t)__builtin_amdgcn_ds_swizzle((int)v, am | (5 << 5));
+            case 6: return (uint32_t)__builtin_amdgcn_ds_swizzle((int)v, am | (6 << 5));
+            default: return (uint32_t)__builtin_amdgcn_ds_swizzle((int)v, am | (7 << 5));
+        }
+    }
+}
+__device__ __forceinline__ float as_float(uint32_t u) { return __uint_as_float(u); }
+
+// Offset given to a masked buffer load: past every table (tables hold at most
+// kOobOffset bytes, and a column offset below 4 KiB keeps it past them).
+constexpr uint32_t kOobOffset = 0xFFFFF000u;
+
+// Fiber-level factoring: the rows of the levels between the root and the leaf
+// are read once per fiber segment of a group run instead of once per nonzero.
+// Off for wide ranks (the per-block partials would not fit the registers),
+// for one-lane groups (a run of one nonzero) and for TTM (no upper levels).
+template <int G, int NB, int OP>
+constexpr bool kFactorOf = OP != kTtm && NB <= 2 && G >= 2;
+
 template <int N, int G, int NB, int TB, typename T, int VEC, int OP>
 __global__ __launch_bounds__(256, (OP == kMttkrp && N == 3 && NB == 1 && G * VEC <= 32) ? 3 : 2) void k_walk(WalkArgs a) {
     static_assert(G == 1 || G == 2 || G == 4 || G == 8, "row groups of 1..8 lanes");
@@ -109,10 +170,11 @@ __global__ __launch_bounds__(256, (OP == kMttkrp && N == 3 && NB == 1 && G * VEC
     // multiplies the rows of the upper levels; rows of `out` are keyed by
     // slice (by leaf fiber for TTM, which has no upper levels)
     constexpr bool kUpper = OP != kTtm;
+    constexpr bool kFactor = kFactorOf<G, NB, OP>;
     // residuals / weights are addressed by tensor-order position
     constexpr bool kPos = OP == kLossResid || OP == kWeightGrad;
-    constexpr int GPR = 64 / G;  // rows per load instruction
-    constexpr int STEPS = G;     // load instructions per chunk and column block
+    constexpr int GPR = 64 / G;  // groups (runs) per chunk
+    constexpr int STEPS = G;     // nonzeros per run = load instructions per chunk and column block
     constexpr int BW = G * VEC;  // columns per block
     constexpr int LF = N - 2;
     constexpr uint32_t kNone = 0xffffffffu;
@@ -150,6 +212,9 @@ __global__ __launch_bounds__(256, (OP == kMttkrp && N == 3 && NB == 1 && G * VEC
     const T *U1 = static_cast<const T *>(a.U[1]);
     const T *U2 = static_cast<const T *>(a.U[2]);
     const T *UL = static_cast<const T *>(a.U[N - 1]);
+    [[maybe_unused]] const __amdgpu_buffer_rsrc_t rs0 = table_rsrc(a.U[0], a.tabBytes[0]);
+    [[maybe_unused]] const __amdgpu_buffer_rsrc_t rs1 = table_rsrc(a.U[1], a.tabBytes[1]);
+    [[maybe_unused]] const __amdgpu_buffer_rsrc_t rs2 = table_rsrc(a.U[2], a.tabBytes[2]);
     const uint32_t *__restrict__ ptrL = a.ptr[LF];
 
     // leaf fiber (and 4-D level-1 entry) of the nonzero before the run; the
@@ -162,7 +227,8 @@ __global__ __launch_bounds__(256, (OP == kMttkrp && N == 3 && NB == 1 && G * VEC
         j0 = wave_upper(a.ptr[1], a.cnt[1], f0);
         jcur = fcur == kNone ? kNone : (a.ptr[1][j0] == f0 && fcur != f0 ? j0 - 1 : j0);
     }
-    // the open slice (leaf fiber for TTM): its partial row is spread over acc of all groups
+    // the open slice (leaf fiber for TTM): its partial row from earlier
+    // chunks is the carry `acc`, summed over the groups' shares
     uint32_t scur = OP == kTtm ? f0 : a.slc[j0];
     // whether the open slice began inside this wave's run: then every
     // contribution to it passes through this wave, and its row is stored
@@ -176,7 +242,7 @@ __global__ __launch_bounds__(256, (OP == kMttkrp && N == 3 && NB == 1 && G * VEC
         for (int e = 0; e < VEC; ++e) acc[b][e] = 0.f;
     double sq = 0.0;
 
-    // row writers: VEC columns of block b from this lane
+    // row writer: VEC columns of block b from this lane
     auto put = [&](int b, uint32_t row, const float *v, bool store) {
         const int col = b * BW + k * VEC;
         if (row == kNone || col >= R) return;
@@ -189,20 +255,6 @@ __global__ __launch_bounds__(256, (OP == kMttkrp && N == 3 && NB == 1 && G * VEC
             for (int e = 0; e < VEC; ++e) unsafeAtomicAdd(o + e, v[e]);
         }
     };
-    // sum of acc over the groups, left in every lane
-    auto reduce = [&](int b) {
-#pragma unroll
-        for (int off = G; off < 64; off <<= 1)
-#pragma unroll
-            for (int e = 0; e < VEC; ++e) acc[b][e] += __shfl_xor(acc[b][e], off, 64);
-    };
-    // write the open partial row of block b to slice s and clear it (all lanes)
-    auto flush = [&](int b, uint32_t s, bool store) {
-        reduce(b);
-        if (g == 0) put(b, s, acc[b], store);
-#pragma unroll
-        for (int e = 0; e < VEC; ++e) acc[b][e] = 0.f;
-    };
 
     // Software pipeline over the chunks of the run.  A chunk's entries come
     // from three dependent reads: its words (leaf index, value, fiber-start
@@ -211,20 +263,27 @@ __global__ __launch_bounds__(256, (OP == kMttkrp && N == 3 && NB == 1 && G * VEC
     // chunk c+2 are read while the rows of chunk c are in flight, so a chunk
     // waits for one read latency instead of three.  Every read in the loop is
     // unconditional (indices clamped into their arrays, validity applied
-    // where the values are used): the in-order load counter then lets each
+    // where the values are used; rows that a lane does not need are read as
+    // out-of-range buffer loads): the in-order load counter then lets each
     // wait cover exactly the reads it needs.
     struct Words {
-        uint32_t lw, word, ps;
+        uint32_t lw, ps;
+        uint64_t word;  // fiber-start bits of the chunk's 64 nonzeros (wave-uniform)
         float v;
     };
     struct Entries {
         uint32_t s, i1, i2;
     };
     const uint32_t lastX = a.nnz - 1;  // nnz >= 1: the run is not empty
+    const uint32_t lastC = chunks - 1;
     auto readWords = [&](uint32_t c, Words &w) {
         const uint32_t x = std::min(c * 64 + (uint32_t)lane, lastX);
         w.lw = a.leaf[x];
-        w.word = a.fbit[x >> 5];
+        // one 8-byte word per chunk, read through the scalar cache (the chunk
+        // index is wave-uniform; chunks past the tensor read the last word)
+        const uint32_t cw = __builtin_amdgcn_readfirstlane(std::min(c, lastC));
+        using ConstU64 = const __attribute__((address_space(4))) uint64_t;
+        w.word = *reinterpret_cast<ConstU64 *>(reinterpret_cast<uintptr_t>(a.fbit + 2 * cw));
         w.ps = 0;
         if constexpr (kPos) w.ps = a.pos[x];
         w.v = 0.f;
@@ -237,7 +296,7 @@ __global__ __launch_bounds__(256, (OP == kMttkrp && N == 3 && NB == 1 && G * VEC
     [[maybe_unused]] const uint32_t lastF = a.cnt[LF] - 1, lastJ = a.cnt[1] - 1;
     auto readEntries = [&](uint32_t c, Words &w, Entries &q) {
         const uint32_t x = c * 64 + (uint32_t)lane;
-        const bool start = x < a.nnz && ((w.word >> (x & 31)) & 1u);
+        const bool start = x < a.nnz && ((w.word >> lane) & 1ull);
         const uint64_t fm = __ballot(start);
         const uint32_t f = fB + (uint32_t)__popcll(fm & low_bits((uint32_t)lane + 1));
         const uint32_t fnext = fB + (uint32_t)__popcll(fm);
@@ -266,81 +325,174 @@ __global__ __launch_bounds__(256, (OP == kMttkrp && N == 3 && NB == 1 && G * VEC
     readEntries(cb, w0, e0);
     readWords(cb + 1, w1);
 
+    // Carried from the previous chunk (every lane alike): the level-1 /
+    // level-2 indices of its last nonzero, and (fiber-level factoring) the
+    // level-1 / level-2 / root rows in effect there.
+    uint32_t i1c = kNone, i2c = kNone;
+    [[maybe_unused]] Raw<T, VEC> cw0[NB], cw1[NB], cw2[NB];
+
     // one chunk: wc / ec hold chunk c; the entries of chunk c+1 (words wn)
-    // go to en, the words of chunk c+2 to wf
+    // go to en, the words of chunk c+2 to wf.  Group g takes the run of
+    // nonzeros g*STEPS .. g*STEPS + STEPS-1 of the chunk, one per step; lane
+    // g*G + t holds the entry of step t (STEPS == G).
     auto step = [&](uint32_t c, Words &wc, Entries &ec, Words &wn, Entries &en,
                     Words &wf) __attribute__((always_inline)) {
         const uint32_t x = c * 64 + (uint32_t)lane;
         const bool valid = x < a.nnz;
         // lanes past the tensor: zero value, no slice; their (clamped) row
         // indices stay valid
-        const uint32_t lw = wc.lw, ps = wc.ps, i1 = ec.i1, i2 = ec.i2;
         const uint32_t s = valid ? ec.s : kNone;
         const float v = valid ? wc.v : 0.f;
         uint32_t sprev = __shfl_up(s, 1, 64);
         if (lane == 0) sprev = scur;
-        const uint64_t bm = kGrad ? __ballot(s != sprev) : 0ull;  // slice starts in the chunk
-        const uint32_t nvalid = std::min<uint32_t>(64, a.nnz - c * 64);
-        // Chunks with several slice starts (short slices) give every group a
-        // contiguous run of entries instead of every GPR-th entry; slices are
-        // then closed group by group and joined across groups by a segmented
-        // scan (below) instead of a cross-group reduction per slice.
-        const bool dense = kGrad && __popcll(bm) >= 3;
-
-        // the entries of this lane's group, one per step
-        // (byte offsets of their rows, computed once by the entry's own lane)
-        uint32_t li[STEPS], l1[STEPS], l2[STEPS], l0[STEPS], pp[STEPS];
-        // entry of step t for this lane's group
-        const uint32_t peB = dense ? (uint32_t)(g * STEPS) : (uint32_t)g;
-        const uint32_t peS = dense ? 1u : (uint32_t)GPR;
-        auto pe = [&](int t) -> uint32_t { return peB + (uint32_t)t * peS; };
-        // slice and value of step t, taken from their lanes where used
-        auto svAt = [&](int t) -> uint32_t { return __shfl(s, (int)pe(t), 64); };
-        auto vvAt = [&](int t) -> float { return __shfl(v, (int)pe(t), 64); };
-#pragma unroll
-        for (int t = 0; t < STEPS; ++t) {
-            const int p = (int)pe(t);
-            li[t] = __shfl(lw * rowBytes, p, 64);
-            if constexpr (kUpper) l1[t] = __shfl(i1 * rowBytes, p, 64);
-            if constexpr (N == 4 && kUpper) l2[t] = __shfl(i2 * rowBytes, p, 64);
-            if constexpr (kModel) l0[t] = __shfl((valid ? s : 0u) * rowBytes, p, 64);
-            if constexpr (OP == kLossResid) pp[t] = __shfl(ps, p, 64);
+        const bool sstart = s != sprev;
+        const uint64_t bm = __ballot(sstart);  // slice starts in the chunk
+        // Segments: maximal runs of nonzeros with the same slice and level-1
+        // index (newJ) and, 4-D, the same level-2 index (newF).  A fiber cut by
+        // the -f split continues its segment.  The rows of a level are read
+        // only where one of its segments opens; lanes of a group whose run
+        // begins inside the segment carried over from the previous chunk take
+        // the carried row.
+        uint64_t FM = 0, JM = 0;
+        if constexpr (kFactor) {
+            uint32_t p1 = __shfl_up(ec.i1, 1, 64);
+            if (lane == 0) p1 = i1c;
+            const bool newJ = sstart || ec.i1 != p1;
+            bool newF = newJ;
+            if constexpr (N == 4) {
+                uint32_t p2 = __shfl_up(ec.i2, 1, 64);
+                if (lane == 0) p2 = i2c;
+                newF = newJ || ec.i2 != p2;
+            }
+            FM = __ballot(newF);
+            JM = N == 4 ? __ballot(newJ) : FM;
         }
-
-        // dense chunks: per group, the partial of the slice open at the run
-        // start (hd, until the first slice start inside the run) and of the
-        // current slice (la)
-        float hd[NB][VEC], la[NB][VEC];
-        bool seen = false;  // a slice started inside this group's run
-        uint32_t ls = kNone;
-        if constexpr (kGrad) {
-            if (dense) {
-                // acc becomes the carry: the partial row of scur from earlier
-                // chunks, summed over the groups (left in every lane)
+        // root-row segments (CP model value): slices; before the first chunk
+        // of the run no root row is carried
+        const uint64_t RM = bm | (i1c == kNone ? 1ull : 0ull);
+        const uint32_t gs = (uint32_t)(g * STEPS);
+        // start bits of the group's steps (bit t: step t)
+        const uint32_t gF = (uint32_t)(FM >> gs);
+        const uint32_t gJ = (uint32_t)(JM >> gs);
+        const uint32_t gS = (uint32_t)(bm >> gs);
+        // the group's first entry still lies in the segment carried over
+        const bool carryF = (FM & low_bits(gs + 1)) == 0;
+        const bool carryJ = (JM & low_bits(gs + 1)) == 0;
+        const bool carryR = (RM & low_bits(gs + 1)) == 0;
+        // steps at which some group reads rows (wave-uniform; bit t: step t)
+        auto fold = [&](uint64_t m, uint64_t m0) -> uint32_t {
+            uint32_t r = 0;
 #pragma unroll
-                for (int b = 0; b < NB; ++b) {
-                    reduce(b);
+            for (int q = 0; q < GPR; ++q) r |= (uint32_t)(m >> (q * STEPS));
+            // step 0: every group whose run does not begin in the carried segment
+            return (r & ~1u & (uint32_t)low_bits(STEPS)) | ((m0 & low_bits((GPR - 1) * STEPS + 1)) ? 1u : 0u);
+        };
+        const uint32_t need1 = kFactor ? fold(N == 3 ? FM : JM, N == 3 ? FM : JM) : 0u;
+        const uint32_t need2 = kFactor && N == 4 ? fold(FM, FM) : 0u;
+        const uint32_t need0 = kFactor && kModel ? fold(RM, RM) : 0u;
+        if constexpr (kFactor) {
+            // the carried rows sit in the last group's lanes: spread them when
+            // some run begins inside a carried segment
+            constexpr int last = (GPR - 1) * G;
+            if (!(((N == 3 ? FM : JM)) & 1ull)) {
 #pragma unroll
-                    for (int e = 0; e < VEC; ++e) hd[b][e] = la[b][e] = 0.f;
+                for (int b = 0; b < NB; ++b) cw1[b].from_lane(cw1[b], last + k);
+            }
+            if constexpr (N == 4) {
+                if (!(FM & 1ull)) {
+#pragma unroll
+                    for (int b = 0; b < NB; ++b) cw2[b].from_lane(cw2[b], last + k);
                 }
-                ls = svAt(0);
+            }
+            if constexpr (kModel) {
+                if (!(RM & 1ull)) {
+#pragma unroll
+                    for (int b = 0; b < NB; ++b) cw0[b].from_lane(cw0[b], last + k);
+                }
             }
         }
-        uint32_t sc0 = scur;
+        // per-lane offsets and values, taken by the group's lanes step by step;
+        // rows a lane does not need read as out-of-range (zero) buffer loads
+        const bool runStart = (lane % STEPS) == 0;
+        const uint32_t oL = wc.lw * rowBytes;
+        const uint32_t o1 = !kFactor ? ec.i1 * rowBytes
+                            : ((runStart && !(N == 3 ? carryF : carryJ)) || (!runStart && ((N == 3 ? FM : JM) >> lane) & 1ull))
+                                ? ec.i1 * rowBytes : kOobOffset;
+        const uint32_t o2 = !kFactor ? ec.i2 * rowBytes
+                            : ((runStart && !carryF) || (!runStart && (FM >> lane) & 1ull)) ? ec.i2 * rowBytes : kOobOffset;
+        const uint32_t o0 = (!kFactor || (runStart && !carryR) || (!runStart && sstart)) ? (valid ? s : 0u) * rowBytes
+                                                                                         : kOobOffset;
+
+        // Partial sums of the group run (per column block):
+        //   fa  leaf-fiber partial  sum co * U_leaf[leaf] over the open leaf fiber
+        //   ga  (4-D) level-1 partial  sum fa (.) U_2[fiber] over its closed fibers
+        //   la  slice partial  (products of closed segments with their rows)
+        //   hd  partial of the slice open at the run start, once a later slice starts
+        // fr1 / fr2 / fr0: the level-1 / level-2 / root rows of the open
+        // segments (fiber-level factoring).
+        float hd[NB][VEC], la[NB][VEC], fa[NB][VEC], ga[NB][VEC];
+        Raw<T, VEC> fr0[NB], fr1[NB], fr2[NB];
+#pragma unroll
+        for (int b = 0; b < NB; ++b)
+#pragma unroll
+            for (int e = 0; e < VEC; ++e) hd[b][e] = la[b][e] = fa[b][e] = ga[b][e] = 0.f;
+        bool seen = false;  // a slice started inside this group's run
+        const uint32_t sFirst = grp_lane<G>(s, 0);
+        uint32_t ls = sFirst;
+        // close the slice open before step tt (all lanes of a group agree)
+        auto closeSlice = [&](int b) {
+            if (!seen) {
+#pragma unroll
+                for (int e = 0; e < VEC; ++e) hd[b][e] = la[b][e];
+            } else {
+                put(b, ls, la[b], true);  // began and ended in this run
+            }
+#pragma unroll
+            for (int e = 0; e < VEC; ++e) la[b][e] = 0.f;
+        };
 #pragma unroll
         for (int t0 = 0; t0 < STEPS; t0 += TB) {
-            Raw<T, VEC> rw[TB][NB], w1[TB][NB], w2[TB][NB], w0[TB][NB];
+            Raw<T, VEC> rw[TB][NB], w1[TB][NB], w2[TB][NB], wz[TB][NB];
 #pragma unroll
-            for (int t = 0; t < TB; ++t)
+            for (int t = 0; t < TB; ++t) {
+                const int tt = t0 + t;
+                const uint32_t li = grp_lane<G>(oL, tt);
+                [[maybe_unused]] uint32_t l1 = 0, l2 = 0, l0 = 0;
+                if constexpr (kUpper) l1 = grp_lane<G>(o1, tt);
+                if constexpr (kUpper && N == 4) l2 = grp_lane<G>(o2, tt);
+                if constexpr (kModel) l0 = grp_lane<G>(o0, tt);
 #pragma unroll
                 for (int b = 0; b < NB; ++b) {
                     // blocks past R read columns 0.. and are dropped where used
                     const uint32_t cb4 = colBytes[b];
-                    rw[t][b].load(UL, li[t0 + t] + cb4);
-                    if constexpr (kUpper) w1[t][b].load(U1, l1[t0 + t] + cb4);
-                    if constexpr (N == 4 && kUpper) w2[t][b].load(U2, l2[t0 + t] + cb4);
-                    if constexpr (kModel) w0[t][b].load(U0, l0[t0 + t] + cb4);
+                    rw[t][b].load(UL, li + cb4);
+                    if constexpr (kUpper && !kFactor) {
+                        w1[t][b].load(U1, l1 + cb4);
+                        if constexpr (N == 4) w2[t][b].load(U2, l2 + cb4);
+                    }
+                    if constexpr (kModel && !kFactor) wz[t][b].load(U0, l0 + cb4);
                 }
+                if constexpr (kFactor) {
+                    // upper-level rows only at steps where some group opens a
+                    // segment (wave-uniform branches)
+                    if ((need1 >> tt) & 1u) {
+#pragma unroll
+                        for (int b = 0; b < NB; ++b) w1[t][b].loadb(rs1, l1 + colBytes[b]);
+                    }
+                    if constexpr (N == 4) {
+                        if ((need2 >> tt) & 1u) {
+#pragma unroll
+                            for (int b = 0; b < NB; ++b) w2[t][b].loadb(rs2, l2 + colBytes[b]);
+                        }
+                    }
+                    if constexpr (kModel) {
+                        if ((need0 >> tt) & 1u) {
+#pragma unroll
+                            for (int b = 0; b < NB; ++b) wz[t][b].loadb(rs0, l0 + colBytes[b]);
+                        }
+                    }
+                }
+            }
             if (t0 == 0) {
                 // issued after this batch's rows and before their use, so the
                 // rows can be waited for without waiting for these reads
@@ -349,11 +501,93 @@ __global__ __launch_bounds__(256, (OP == kMttkrp && N == 3 && NB == 1 && G * VEC
                 readWords(c + 2, wf);
                 __builtin_amdgcn_sched_barrier(0);
             }
-            // coefficient of every entry of the batch
-            float co[TB];
 #pragma unroll
             for (int t = 0; t < TB; ++t) {
-                const uint32_t p = pe(t0 + t);
+                const int tt = t0 + t;
+                const float vt = as_float(grp_lane<G>(__float_as_uint(v), tt));
+                [[maybe_unused]] uint32_t pt = 0;
+                if constexpr (OP == kLossResid) pt = grp_lane<G>(wc.ps, tt);
+                const bool sst = tt > 0 && ((gS >> tt) & 1u);  // a slice starts here
+                [[maybe_unused]] uint32_t sp = 0;
+                if constexpr (kGrad) sp = grp_lane<G>(s, tt);
+                // segment starts: close the open segments with their rows, take
+                // the rows of the new ones
+                if constexpr (kFactor) {
+                    if (tt == 0) {
+                        // the run's first rows: loaded, or carried over
+#pragma unroll
+                        for (int b = 0; b < NB; ++b) {
+                            fr1[b] = cw1[b];
+                            if (need1 & 1u)
+                                if (!(N == 3 ? carryF : carryJ)) fr1[b] = w1[t][b];
+                            if constexpr (N == 4) {
+                                fr2[b] = cw2[b];
+                                if (need2 & 1u)
+                                    if (!carryF) fr2[b] = w2[t][b];
+                            }
+                            if constexpr (kModel) {
+                                fr0[b] = cw0[b];
+                                if (need0 & 1u)
+                                    if (!carryR) fr0[b] = wz[t][b];
+                            }
+                        }
+                    } else if ((gF >> tt) & 1u) {
+#pragma unroll
+                        for (int b = 0; b < NB; ++b) {
+                            float q1[VEC];
+                            fr1[b].get(q1);
+                            if constexpr (N == 3) {
+                                if constexpr (kGrad) {
+#pragma unroll
+                                    for (int e = 0; e < VEC; ++e) {
+                                        la[b][e] = fmaf(fa[b][e], q1[e], la[b][e]);
+                                        fa[b][e] = 0.f;
+                                    }
+                                    if (sst) closeSlice(b);
+                                }
+                                fr1[b] = w1[t][b];
+                            } else {
+                                float q2[VEC];
+                                fr2[b].get(q2);
+                                if constexpr (kGrad) {
+#pragma unroll
+                                    for (int e = 0; e < VEC; ++e) {
+                                        ga[b][e] = fmaf(fa[b][e], q2[e], ga[b][e]);
+                                        fa[b][e] = 0.f;
+                                    }
+                                }
+                                if ((gJ >> tt) & 1u) {
+                                    if constexpr (kGrad) {
+#pragma unroll
+                                        for (int e = 0; e < VEC; ++e) {
+                                            la[b][e] = fmaf(ga[b][e], q1[e], la[b][e]);
+                                            ga[b][e] = 0.f;
+                                        }
+                                        if (sst) closeSlice(b);
+                                    }
+                                    fr1[b] = w1[t][b];
+                                }
+                                fr2[b] = w2[t][b];
+                            }
+                            if constexpr (kModel)
+                                if (sst) fr0[b] = wz[t][b];
+                        }
+                    }
+                } else if constexpr (kGrad) {
+                    if (sst) {
+#pragma unroll
+                        for (int b = 0; b < NB; ++b) closeSlice(b);
+                    }
+                }
+                if constexpr (kGrad) {
+                    if (sst) {
+                        ls = sp;
+                        seen = true;
+                    }
+                }
+                // coefficient of the entry
+                const bool ok = c * 64 + gs + (uint32_t)tt < a.nnz;
+                float co;
                 if constexpr (kModel) {
                     float d = 0.f;
 #pragma unroll
@@ -362,12 +596,18 @@ __global__ __launch_bounds__(256, (OP == kMttkrp && N == 3 && NB == 1 && G * VEC
                         if (col < R) {
                             float r[VEC], q[VEC], z[VEC];
                             rw[t][b].get(r);
-                            w1[t][b].get(q);
-                            w0[t][b].get(z);
+                            if constexpr (kFactor) {
+                                fr1[b].get(q);
+                                fr0[b].get(z);
+                            } else {
+                                w1[t][b].get(q);
+                                wz[t][b].get(z);
+                            }
 #pragma unroll
                             for (int e = 0; e < VEC; ++e) r[e] *= q[e] * z[e];
                             if constexpr (N == 4) {
-                                w2[t][b].get(q);
+                                if constexpr (kFactor) fr2[b].get(q);
+                                else w2[t][b].get(q);
 #pragma unroll
                                 for (int e = 0; e < VEC; ++e) r[e] *= q[e];
                             }
@@ -377,125 +617,92 @@ __global__ __launch_bounds__(256, (OP == kMttkrp && N == 3 && NB == 1 && G * VEC
                     }
 #pragma unroll
                     for (int off = 1; off < G; off <<= 1) d += __shfl_xor(d, off, 64);
-                    const bool ok = p < nvalid;
-                    const float vt = vvAt(t0 + t);  // every lane takes part
                     const float e = ok ? d - vt : 0.f;
                     if (k == 0 && ok) {
                         sq += (double)e * (double)e;
                         if constexpr (OP == kLossResid)
-                            if (a.resid) a.resid[pp[t0 + t]] = e;
+                            if (a.resid) a.resid[pt] = e;
                     }
-                    co[t] = e * a.alpha;
+                    co = e * a.alpha;
                 } else if constexpr (OP == kWeightGrad) {
-                    co[t] = vvAt(t0 + t) * a.alpha;
+                    co = vt * a.alpha;
                 } else {
-                    co[t] = vvAt(t0 + t);
+                    co = vt;
                 }
-            }
-            if constexpr (kGrad) {
-                const uint32_t plo = (uint32_t)(t0 * GPR), phi = (uint32_t)((t0 + TB) * GPR);
-                const uint64_t bmb = dense ? 0ull : bm & low_bits(phi) & ~low_bits(plo);
-                const uint32_t ls0 = ls;
-                const bool seen0 = seen;
+                if constexpr (kGrad) {
 #pragma unroll
-                for (int b = 0; b < NB; ++b) {
-                    // product of the entry's rows; the coefficient co is
-                    // applied by the fused multiply-add that sums it.  Lanes
-                    // of columns past R compute on column-0 data and are
-                    // never written.
-                    float cf[TB][VEC];
+                    for (int b = 0; b < NB; ++b) {
+                        float r[VEC];
+                        rw[t][b].get(r);
+                        if constexpr (kFactor) {
 #pragma unroll
-                    for (int t = 0; t < TB; ++t) {
-                        rw[t][b].get(cf[t]);
-                        if constexpr (kUpper) {
-                            float q[VEC];
-                            w1[t][b].get(q);
+                            for (int e = 0; e < VEC; ++e) fa[b][e] = fmaf(co, r[e], fa[b][e]);
+                        } else {
+                            if constexpr (kUpper) {
+                                float q[VEC];
+                                w1[t][b].get(q);
 #pragma unroll
-                            for (int e = 0; e < VEC; ++e) cf[t][e] *= q[e];
-                            if constexpr (N == 4) {
-                                w2[t][b].get(q);
+                                for (int e = 0; e < VEC; ++e) r[e] *= q[e];
+                                if constexpr (N == 4) {
+                                    w2[t][b].get(q);
 #pragma unroll
-                                for (int e = 0; e < VEC; ++e) cf[t][e] *= q[e];
-                            }
-                        }
-                    }
-                    if (dense) {
-                        uint32_t lsb = ls0;
-                        bool sb = seen0;
-#pragma unroll
-                        for (int t = 0; t < TB; ++t) {
-                            const uint32_t sp = svAt(t0 + t);
-                            if (sp != lsb) {
-                                if (!sb) {
-#pragma unroll
-                                    for (int e = 0; e < VEC; ++e) hd[b][e] = la[b][e];
-                                } else {
-                                    put(b, lsb, la[b], true);  // began and ended in this run
+                                    for (int e = 0; e < VEC; ++e) r[e] *= q[e];
                                 }
-#pragma unroll
-                                for (int e = 0; e < VEC; ++e) la[b][e] = 0.f;
-                                lsb = sp;
-                                sb = true;
                             }
 #pragma unroll
-                            for (int e = 0; e < VEC; ++e) la[b][e] = fmaf(co[t], cf[t][e], la[b][e]);
-                        }
-                        ls = lsb;
-                        seen = sb;
-                    } else if (bmb == 0) {
-#pragma unroll
-                        for (int t = 0; t < TB; ++t)
-#pragma unroll
-                            for (int e = 0; e < VEC; ++e) acc[b][e] = fmaf(co[t], cf[t][e], acc[b][e]);
-                    } else {
-                        // slice segments of the batch in order; wave-uniform loop
-                        uint64_t rest = bmb;
-                        uint32_t lo = plo, sc = sc0;
-                        bool in = inside;
-                        while (rest) {
-                            const uint32_t hi = (uint32_t)__builtin_ctzll(rest);
-                            rest &= rest - 1;
-#pragma unroll
-                            for (int t = 0; t < TB; ++t) {
-                                const uint32_t p = pe(t0 + t);
-                                if (p >= lo && p < hi)
-#pragma unroll
-                                    for (int e = 0; e < VEC; ++e) acc[b][e] = fmaf(co[t], cf[t][e], acc[b][e]);
-                            }
-                            flush(b, sc, in);
-                            sc = __shfl(s, (int)hi, 64);
-                            lo = hi;
-                            in = true;
-                        }
-#pragma unroll
-                        for (int t = 0; t < TB; ++t) {
-                            const uint32_t p = pe(t0 + t);
-                            if (p >= lo)
-#pragma unroll
-                                for (int e = 0; e < VEC; ++e) acc[b][e] = fmaf(co[t], cf[t][e], acc[b][e]);
+                            for (int e = 0; e < VEC; ++e) la[b][e] = fmaf(co, r[e], la[b][e]);
                         }
                     }
-                }
-                if (bmb) {
-                    sc0 = __shfl(s, 63 - (int)__builtin_clzll(bmb), 64);
-                    inside = true;
                 }
             }
         }
         if constexpr (kGrad) {
-            if (dense) {
-                // Join the slices across the groups.  Group g links to group
-                // g-1 when no slice starts at its first entry; a group without
-                // a slice start inside its run passes the chain on.  The
-                // segmented inclusive scan over groups gives every group the
-                // total of the slice open at its run end (op); rt: that chain
-                // runs back to the chunk start (and then carries the partial
-                // row of scur from earlier chunks).
-                const uint32_t gs = (uint32_t)(g * STEPS);
-                const bool cont = !((bm >> gs) & 1ull);  // no slice starts at the run start
+            if constexpr (kFactor) {
+                // close the segment open at the run end with its rows
+#pragma unroll
+                for (int b = 0; b < NB; ++b) {
+                    float q1[VEC];
+                    fr1[b].get(q1);
+                    if constexpr (N == 4) {
+                        float q2[VEC];
+                        fr2[b].get(q2);
+#pragma unroll
+                        for (int e = 0; e < VEC; ++e) ga[b][e] = fmaf(fa[b][e], q2[e], ga[b][e]);
+#pragma unroll
+                        for (int e = 0; e < VEC; ++e) la[b][e] = fmaf(ga[b][e], q1[e], la[b][e]);
+                    } else {
+#pragma unroll
+                        for (int e = 0; e < VEC; ++e) la[b][e] = fmaf(fa[b][e], q1[e], la[b][e]);
+                    }
+                }
+            }
+            // No slice starts in the chunk: every run belongs to the open
+            // slice, and each group adds its partial to its share of the carry.
+            if (bm == 0) {
+#pragma unroll
+                for (int b = 0; b < NB; ++b)
+#pragma unroll
+                    for (int e = 0; e < VEC; ++e) acc[b][e] += la[b][e];
+            } else {
+                // The carry (the open slice's partial row from earlier chunks),
+                // summed over the groups' shares and left in every lane.
+#pragma unroll
+                for (int b = 0; b < NB; ++b)
+#pragma unroll
+                    for (int off = G; off < 64; off <<= 1)
+#pragma unroll
+                        for (int e = 0; e < VEC; ++e) acc[b][e] += __shfl_xor(acc[b][e], off, 64);
+                // Join the slices across the groups.  Group g links to group g-1
+                // when no slice starts at its first entry; a group without a
+                // slice start inside its run passes the chain on.  The segmented
+                // inclusive scan over groups gives every group the total of the
+                // slice open at its run end (op); rt: that chain runs back to the
+                // chunk start (and then carries the partial row of scur from
+                // earlier chunks, acc).
+                const bool cont = !(gS & 1u);  // no slice starts at the run start
                 const bool single = !seen;
                 bool rt = single && cont;
-                const uint32_t sLast = ls, sFirst = svAt(0);
+                const uint32_t sLast = ls;
                 float op[NB][VEC];
 #pragma unroll
                 for (int b = 0; b < NB; ++b)
@@ -526,9 +733,9 @@ __global__ __launch_bounds__(256, (OP == kMttkrp && N == 3 && NB == 1 && G * VEC
 #pragma unroll
                     for (int b = 0; b < NB; ++b) put(b, scur, acc[b], inside);
                 }
-                // a group with a slice start inside its run closes the slice
-                // open at its run start: its head partial plus the chain ending
-                // at the previous group's run end (the carry for group 0)
+                // a group with a slice start inside its run closes the slice open
+                // at its run start: its head partial plus the chain ending at the
+                // previous group's run end (the carry for group 0)
                 const int rtUp = __shfl_up(rt ? 1 : 0, G, 64);  // every lane takes part
                 const bool back = cont && (g == 0 || rtUp != 0);  // the slice is scur
 #pragma unroll
@@ -542,7 +749,8 @@ __global__ __launch_bounds__(256, (OP == kMttkrp && N == 3 && NB == 1 && G * VEC
                     if (!single) put(b, sFirst, h, !back || inside);
                 }
                 // the slice open at a run end closes there when the next run
-                // starts a slice; the last group's stays open
+                // starts a slice; the last group's stays open and becomes the
+                // carry (held by the last group)
                 const bool nextStarts = g + 1 < GPR && ((bm >> (gs + STEPS)) & 1ull);
                 if (nextStarts) {
 #pragma unroll
@@ -554,10 +762,21 @@ __global__ __launch_bounds__(256, (OP == kMttkrp && N == 3 && NB == 1 && G * VEC
 #pragma unroll
                     for (int e = 0; e < VEC; ++e) acc[b][e] = g == GPR - 1 ? op[b][e] : 0.f;
                 inside = rtLast ? inside : true;
-                sc0 = __shfl(s, 63, 64);
             }
         }
-        scur = sc0;
+        if constexpr (kFactor) {
+            // the rows in effect at the run ends (the last group's: those at
+            // the chunk's last nonzero)
+#pragma unroll
+            for (int b = 0; b < NB; ++b) {
+                cw1[b] = fr1[b];
+                if constexpr (N == 4) cw2[b] = fr2[b];
+                if constexpr (kModel) cw0[b] = fr0[b];
+            }
+            i1c = __shfl(ec.i1, 63, 64);
+            if constexpr (N == 4) i2c = __shfl(ec.i2, 63, 64);
+        }
+        scur = __shfl(s, 63, 64);
     };
     // unrolled by the pipeline depth: the buffers rotate by name, so no
     // register with a read in flight is ever copied (a copy would wait for it)
@@ -571,7 +790,13 @@ __global__ __launch_bounds__(256, (OP == kMttkrp && N == 3 && NB == 1 && G * VEC
     if constexpr (kGrad) {
         // the last slice may continue in the next wave
 #pragma unroll
-        for (int b = 0; b < NB; ++b) flush(b, scur, false);
+        for (int b = 0; b < NB; ++b) {
+#pragma unroll
+            for (int off = G; off < 64; off <<= 1)
+#pragma unroll
+                for (int e = 0; e < VEC; ++e) acc[b][e] += __shfl_xor(acc[b][e], off, 64);
+            if (g == 0) put(b, scur, acc[b], false);
+        }
     }
     if constexpr (kModel) {
         if (a.sumsq) {

@@ -54,7 +54,7 @@ class FormatInfo(C.Structure):
         ("modeOrder", C.c_int * 4), ("nLevel", C.c_uint64 * 4), ("nFibers", C.c_uint64),
         ("droppedSlices", C.c_uint64), ("COOnnz", C.c_uint64), ("CSLnnz", C.c_uint64),
         ("HCSRnnz", C.c_uint64), ("redunMode", C.c_int), ("modeTrees", C.c_int),
-        ("modeLevel", (C.c_uint64 * 4) * 4),
+        ("modeLevel", (C.c_uint64 * 4) * 4), ("deviceBytes", C.c_uint64),
     ]
 
 
@@ -282,6 +282,7 @@ class Format:
             "CSLnnz": i.CSLnnz, "HCSRnnz": i.HCSRnnz, "redunMode": i.redunMode,
             "modeTrees": i.modeTrees,
             "modeLevel": [list(i.modeLevel[m][: i.ndims]) for m in range(i.ndims)],
+            "deviceBytes": i.deviceBytes,
         }
 
     def export(self, name, k=0):
