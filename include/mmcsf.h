@@ -308,6 +308,25 @@ int mmcsf_cpd_als(const mmcsf_format *f, int R, int max_iters, double tol,
                   int mttkrp_dtype, float *const *factors, double *lambda,
                   mmcsf_cpd_stats *stats, void *stream);
 
+/*
+ * CP-ALS exactly as mmcsf_cpd_als, allocating no device memory.  No
+ * counterpart in the reference: for callers that cannot allocate (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The work space -- the float64 Grams, normal-matrix work space and inverse,
+ * lambda / norm vectors, the MTTKRP output of the largest mode (rows x R
+ * float32) and, for bfloat16 MTTKRP, the bfloat16 factor copies, each part
+ * rounded up to 256 bytes -- lives in a device array of the library of
+ * MMCSF_CPD_NOALLOC_SCRATCH_BYTES, one per GPU, held until the process ends;
+ * a larger work space is refused with MMCSF_ERR_UNSUPPORTED before any work.
+ * Calls that overlap on one GPU share that array and race.  No timing
+ * events are created: stats->ms_mttkrp is 0.
+ */
+#define MMCSF_CPD_NOALLOC_SCRATCH_BYTES (64ull << 20)
+int mmcsf_cpd_als_noalloc(const mmcsf_format *f, int R, int max_iters,
+                          double tol, int mttkrp_dtype, float *const *factors,
+                          double *lambda, mmcsf_cpd_stats *stats,
+                          void *stream);
+
 /* G = U^T U (float64 R x R, device) of a device float32 rows x R matrix, with
  * MFMA (v_mfma_f32_16x16x4f32) over row chunks and float64 chunk sums
  * (the ssyrk_ calls of cpd_cpu.cu:543-546, 649-652). */

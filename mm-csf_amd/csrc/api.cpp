@@ -905,19 +905,34 @@ int mmcsf_mttkrp_coo_host(const mmcsf_format *fh, int mode, int R, const float *
     });
 }
 
+namespace {
+void cpd_checks(const mmcsf_format *fh, int R, int max_iters, int dtype, float *const *factors, mmcsf_cpd_stats *stats) {
+    MMCSF_REQUIRE(fh && factors, MMCSF_ERR_ARG, "null argument");
+    const Format &f = fh->f;
+    MMCSF_REQUIRE(f.type == MMCSF_TYPE_MMCSF, MMCSF_ERR_ARG, "CP-ALS runs on an MM-CSF format");
+    MMCSF_REQUIRE(R >= 1 && R <= 256, MMCSF_ERR_UNSUPPORTED, "R must be in 1..256");
+    MMCSF_REQUIRE(dtype == MMCSF_F32 || dtype == MMCSF_BF16, MMCSF_ERR_ARG, "dtype must be F32 or BF16");
+    MMCSF_REQUIRE(f.nnz > 0, MMCSF_ERR_ARG, "CP-ALS needs at least one nonzero");
+    MMCSF_REQUIRE(max_iters >= 1, MMCSF_ERR_ARG, "max_iters must be positive");
+    for (int m = 0; m < f.ndims; ++m) MMCSF_REQUIRE(factors[m] != nullptr, MMCSF_ERR_ARG, "missing factor");
+    if (stats) std::memset(stats, 0, sizeof(*stats));
+}
+}  // namespace
+
 int mmcsf_cpd_als(const mmcsf_format *fh, int R, int max_iters, double tol, int dtype, float *const *factors,
                   double *lambda, mmcsf_cpd_stats *stats, void *stream) {
     return guarded([&] {
-        MMCSF_REQUIRE(fh && factors, MMCSF_ERR_ARG, "null argument");
-        const Format &f = fh->f;
-        MMCSF_REQUIRE(f.type == MMCSF_TYPE_MMCSF, MMCSF_ERR_ARG, "CP-ALS runs on an MM-CSF format");
-        MMCSF_REQUIRE(R >= 1 && R <= 256, MMCSF_ERR_UNSUPPORTED, "R must be in 1..256");
-        MMCSF_REQUIRE(dtype == MMCSF_F32 || dtype == MMCSF_BF16, MMCSF_ERR_ARG, "dtype must be F32 or BF16");
-        MMCSF_REQUIRE(f.nnz > 0, MMCSF_ERR_ARG, "CP-ALS needs at least one nonzero");
-        MMCSF_REQUIRE(max_iters >= 1, MMCSF_ERR_ARG, "max_iters must be positive");
-        for (int m = 0; m < f.ndims; ++m) MMCSF_REQUIRE(factors[m] != nullptr, MMCSF_ERR_ARG, "missing factor");
-        if (stats) std::memset(stats, 0, sizeof(*stats));
-        cpd_als(f, R, max_iters, tol, dtype, factors, lambda, stats, static_cast<hipStream_t>(stream));
+        cpd_checks(fh, R, max_iters, dtype, factors, stats);
+        cpd_als(fh->f, R, max_iters, tol, dtype, factors, lambda, stats, static_cast<hipStream_t>(stream));
+    });
+}
+
+int mmcsf_cpd_als_noalloc(const mmcsf_format *fh, int R, int max_iters, double tol, int dtype,
+                          float *const *factors, double *lambda, mmcsf_cpd_stats *stats, void *stream) {
+    return guarded([&] {
+        static_assert(MMCSF_CPD_NOALLOC_SCRATCH_BYTES == (64ull << 20), "scratch size stated in mmcsf.h");
+        cpd_checks(fh, R, max_iters, dtype, factors, stats);
+        cpd_als_noalloc(fh->f, R, max_iters, tol, dtype, factors, lambda, stats, static_cast<hipStream_t>(stream));
     });
 }
 

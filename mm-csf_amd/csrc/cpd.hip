@@ -10,6 +10,8 @@
 //   the row solve U = M H^-1, column norms and the fit     float64
 #include <algorithm>
 #include <chrono>
+#include <mutex>
+#include <string>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -408,101 +410,139 @@ void gram(const float *U, uint64_t rows, int R, double *G, hipStream_t s) {
     MMCSF_HIP(hipGetLastError());
 }
 
-// CP-ALS driver state
-struct CpdRun {
-    const Format &f;
-    int N, R;
-    hipStream_t s;
-    DevArray<double> G, W, Hinv, lam, lamFinal, sq, scal;
-    DevArray<uint32_t> mx;
-    DevArray<int> status;
-    DevArray<float> M;
-    std::vector<DevArray<uint16_t>> Ub;
-    CpdRun(const Format &fmt, int R_, hipStream_t st) : f(fmt), N(fmt.ndims), R(R_), s(st) {}
+__global__ void k_zero_u32(uint32_t *p, uint64_t n) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = 0u;
+}
+
+// CP-ALS work space: the float64 Grams, normal-matrix work space and inverse,
+// lambda / norm vectors and fit scalars, the MTTKRP output of the mode being
+// updated, and (bfloat16 MTTKRP) the bfloat16 factor copies; every part
+// 256-byte aligned.
+struct CpdSpace {
+    double *G, *W, *Hinv, *lam, *lamFinal, *sq, *scal;
+    uint32_t *mx;
+    int *status;
+    float *M;
+    uint16_t *Ub[kMaxDims];
 };
 
-void cpd_als(const Format &f, int R, int iters, double tol, int dtype, float *const *U, double *lambdaHost,
-             mmcsf_cpd_stats *stats, hipStream_t s) {
+static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+// bytes of the work space, and (base != null) the parts carved from it
+static size_t cpd_space(const Format &f, int R, int dtype, char *base, CpdSpace *sp) {
     const int N = f.ndims;
-    CpdRun st(f, R, s);
     const size_t RR = (size_t)R * R;
     uint64_t maxRows = 0;
     for (int m = 0; m < N; ++m) maxRows = std::max<uint64_t>(maxRows, f.dims[m]);
-    st.G.alloc(N * RR);
-    st.W.alloc(3 * RR);
-    st.Hinv.alloc(RR);
-    st.lam.alloc(R);
-    st.lamFinal.alloc(R);
-    st.sq.alloc(R);
-    st.mx.alloc(R);
-    st.scal.alloc(4);  // [0] |X|^2, [1] inner, [2] fit
-    st.status.alloc(1);
-    st.M.alloc(maxRows * R);
-    if (dtype == MMCSF_BF16) {
-        st.Ub.resize(N);
-        for (int m = 0; m < N; ++m) st.Ub[m].alloc(f.dims[m] * R);
-    }
+    size_t off = 0;
+    auto take = [&](size_t bytes) -> char * {
+        char *p = base ? base + off : nullptr;
+        off += align256(bytes);
+        return p;
+    };
+    CpdSpace t{};
+    t.G = (double *)take(N * RR * sizeof(double));
+    t.W = (double *)take(3 * RR * sizeof(double));
+    t.Hinv = (double *)take(RR * sizeof(double));
+    t.lam = (double *)take(R * sizeof(double));
+    t.lamFinal = (double *)take(R * sizeof(double));
+    t.sq = (double *)take(R * sizeof(double));
+    t.scal = (double *)take(4 * sizeof(double));  // [0] |X|^2, [1] inner, [2] fit
+    t.mx = (uint32_t *)take(R * sizeof(uint32_t));
+    t.status = (int *)take(sizeof(int));
+    t.M = (float *)take(maxRows * R * sizeof(float));
+    for (int m = 0; m < kMaxDims; ++m) t.Ub[m] = nullptr;
+    if (dtype == MMCSF_BF16)
+        for (int m = 0; m < N; ++m) t.Ub[m] = (uint16_t *)take(f.dims[m] * R * sizeof(uint16_t));
+    if (sp) *sp = t;
+    return off;
+}
+
+// The ALS loop on a work space `ws` of cpd_space(...) bytes.  Every part that
+// must start at zero is cleared by a kernel here, so the work space may hold
+// anything on entry.  timeMttkrp: time the MTTKRP launches with HIP events.
+static void cpd_als_on(const Format &f, int R, int iters, double tol, int dtype, float *const *U, double *lambdaHost,
+                       mmcsf_cpd_stats *stats, hipStream_t s, char *ws, bool timeMttkrp) {
+    const int N = f.ndims;
+    const size_t RR = (size_t)R * R;
+    CpdSpace st{};
+    cpd_space(f, R, dtype, ws, &st);
     auto refresh_bf16 = [&](int m) {
         if (dtype == MMCSF_BF16)
             hipLaunchKernelGGL(k_f32_to_bf16, dim3(grid_for(f.dims[m] * R, 1u << 30)), dim3(256), 0, s, U[m],
-                               st.Ub[m].p, (uint64_t)f.dims[m] * R);
+                               st.Ub[m], (uint64_t)f.dims[m] * R);
+    };
+    auto clear_norms = [&]() {
+        hipLaunchKernelGGL(k_zero_d, dim3(1), dim3(256), 0, s, st.sq, (uint64_t)R);
+        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(256), 0, s, st.mx, (uint64_t)R);
     };
     for (int m = 0; m < N; ++m) {
-        gram(U[m], f.dims[m], R, st.G.p + m * RR, s);
+        gram(U[m], f.dims[m], R, st.G + m * RR, s);
         refresh_bf16(m);
     }
-    hipLaunchKernelGGL(k_zero_d, dim3(1), dim3(256), 0, s, st.scal.p, (uint64_t)4);
+    hipLaunchKernelGGL(k_zero_d, dim3(1), dim3(256), 0, s, st.scal, (uint64_t)4);
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, (uint32_t *)st.status, (uint64_t)1);
     hipLaunchKernelGGL(k_sumsq, dim3(grid_for(f.nnz, 4096)), dim3(256), 0, s, f.csf.vals.p, (uint64_t)f.nnz,
-                       st.scal.p);
-    hipLaunchKernelGGL(k_fill_d, dim3(1), dim3(256), 0, s, st.lam.p, R, 1.0);
+                       st.scal);
+    hipLaunchKernelGGL(k_fill_d, dim3(1), dim3(256), 0, s, st.lam, R, 1.0);
     Factors F{};
     F.dtype = dtype;
     F.R = R;
-    for (int m = 0; m < N; ++m) F.U[m] = dtype == MMCSF_BF16 ? (const void *)st.Ub[m].p : (const void *)U[m];
-    hipEvent_t e0, e1;
-    MMCSF_HIP(hipEventCreate(&e0));
-    MMCSF_HIP(hipEventCreate(&e1));
+    for (int m = 0; m < N; ++m) F.U[m] = dtype == MMCSF_BF16 ? (const void *)st.Ub[m] : (const void *)U[m];
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (timeMttkrp) {
+        MMCSF_HIP(hipEventCreate(&e0));
+        MMCSF_HIP(hipEventCreate(&e1));
+    }
+    struct Events {
+        hipEvent_t a, b;
+        ~Events() {
+            if (a) (void)hipEventDestroy(a);
+            if (b) (void)hipEventDestroy(b);
+        }
+    } events{e0, e1};
     double mttkrpMs = 0, oldfit = 0, fit = 0;
     int it = 0, lu = 0;
     auto t0 = std::chrono::steady_clock::now();
     for (it = 0; it < iters; ++it) {
         for (int m = 0; m < N; ++m) {
-            mttkrp_mode(f, m, F, st.M.p, s, e0, e1);
+            mttkrp_mode(f, m, F, st.M, s, e0, e1);
             const size_t ishm = 3 * RR * sizeof(double) <= kInverseLds ? 3 * RR * sizeof(double) : 0;
-            hipLaunchKernelGGL(k_normal_inverse, dim3(1), dim3(256), ishm, s, st.G.p, N, m, R, st.W.p, st.Hinv.p,
-                               st.status.p);
+            hipLaunchKernelGGL(k_normal_inverse, dim3(1), dim3(256), ishm, s, st.G, N, m, R, st.W, st.Hinv,
+                               st.status);
             const size_t shm = RR * sizeof(double) <= 32768 ? RR * sizeof(double) : 0;
-            hipLaunchKernelGGL(k_apply_inverse, dim3(grid_for(f.dims[m] * ((R + 3) / 4), 8192)), dim3(256), shm, s, st.M.p,
-                               st.Hinv.p, (uint32_t)f.dims[m], R, U[m]);
+            hipLaunchKernelGGL(k_apply_inverse, dim3(grid_for(f.dims[m] * ((R + 3) / 4), 8192)), dim3(256), shm, s, st.M,
+                               st.Hinv, (uint32_t)f.dims[m], R, U[m]);
             const int kind = it == 0 ? 0 : 1;  // 2-norm at iteration 0, max-norm after
-            MMCSF_HIP(hipMemsetAsync(st.sq.p, 0, R * sizeof(double), s));
-            MMCSF_HIP(hipMemsetAsync(st.mx.p, 0, R * sizeof(uint32_t), s));
+            clear_norms();
             hipLaunchKernelGGL(k_colnorm, dim3(colnorm_grid(f.dims[m], R)), dim3(256), 0, s, U[m],
-                               (uint32_t)f.dims[m], R, kind, st.sq.p, st.mx.p);
-            hipLaunchKernelGGL(k_lambda, dim3(1), dim3(std::max(R, 64)), 0, s, R, kind, st.sq.p, st.mx.p, st.lam.p);
+                               (uint32_t)f.dims[m], R, kind, st.sq, st.mx);
+            hipLaunchKernelGGL(k_lambda, dim3(1), dim3(std::max(R, 64)), 0, s, R, kind, st.sq, st.mx, st.lam);
             hipLaunchKernelGGL(k_scale_cols, dim3(grid_for(f.dims[m] * R, 8192)), dim3(256), 0, s, U[m],
-                               (uint32_t)f.dims[m], R, st.lam.p, (double *)nullptr);
-            gram(U[m], f.dims[m], R, st.G.p + m * RR, s);
+                               (uint32_t)f.dims[m], R, st.lam, (double *)nullptr);
+            gram(U[m], f.dims[m], R, st.G + m * RR, s);
             refresh_bf16(m);
             if (m == N - 1) {
                 // inner product of the last MTTKRP with the updated last factor
-                MMCSF_HIP(hipMemsetAsync(st.scal.p + 1, 0, sizeof(double), s));
-                hipLaunchKernelGGL(k_dot_lambda, dim3(grid_for(f.dims[m] * R, 4096)), dim3(256), 0, s, U[m], st.M.p,
-                                   (uint32_t)f.dims[m], R, st.lam.p, st.scal.p + 1);
+                hipLaunchKernelGGL(k_zero_d, dim3(1), dim3(64), 0, s, st.scal + 1, (uint64_t)1);
+                hipLaunchKernelGGL(k_dot_lambda, dim3(grid_for(f.dims[m] * R, 4096)), dim3(256), 0, s, U[m], st.M,
+                                   (uint32_t)f.dims[m], R, st.lam, st.scal + 1);
             }
+            MMCSF_HIP(hipGetLastError());
             int stat = 0;
-            MMCSF_HIP(hipMemcpyAsync(&stat, st.status.p, sizeof(int), hipMemcpyDeviceToHost, s));
-            MMCSF_HIP(hipEventSynchronize(e1));
-            float ms = 0;
-            MMCSF_HIP(hipEventElapsedTime(&ms, e0, e1));
-            mttkrpMs += ms;
+            MMCSF_HIP(hipMemcpyAsync(&stat, st.status, sizeof(int), hipMemcpyDeviceToHost, s));
             MMCSF_HIP(hipStreamSynchronize(s));
+            if (timeMttkrp) {
+                float ms = 0;
+                MMCSF_HIP(hipEventElapsedTime(&ms, e0, e1));
+                mttkrpMs += ms;
+            }
             if (stat == 1) ++lu;
             MMCSF_REQUIRE(stat != 2, MMCSF_ERR_UNSUPPORTED, "normal equations are singular");
         }
-        hipLaunchKernelGGL(k_fit, dim3(1), dim3(256), 0, s, st.G.p, N, R, st.lam.p, st.scal.p + 1, st.scal.p,
-                           st.scal.p + 2);
-        MMCSF_HIP(hipMemcpyAsync(&fit, st.scal.p + 2, sizeof(double), hipMemcpyDeviceToHost, s));
+        hipLaunchKernelGGL(k_fit, dim3(1), dim3(256), 0, s, st.G, N, R, st.lam, st.scal + 1, st.scal, st.scal + 2);
+        MMCSF_HIP(hipMemcpyAsync(&fit, st.scal + 2, sizeof(double), hipMemcpyDeviceToHost, s));
         MMCSF_HIP(hipStreamSynchronize(s));
         if (stats && it < (int)(sizeof(stats->fit_history) / sizeof(double))) stats->fit_history[it] = fit;
         if (it > 0 && std::fabs(fit - oldfit) < tol) {  // cpd_cpu.cu:670-672
@@ -513,20 +553,19 @@ void cpd_als(const Format &f, int R, int iters, double tol, int dtype, float *co
     }
     auto t1 = std::chrono::steady_clock::now();
     // GetFinalLambda (cpd_cpu.cu:333-349): fold the column 2-norms into lambda
-    MMCSF_HIP(hipMemcpyAsync(st.lamFinal.p, st.lam.p, R * sizeof(double), hipMemcpyDeviceToDevice, s));
+    MMCSF_HIP(hipMemcpyAsync(st.lamFinal, st.lam, R * sizeof(double), hipMemcpyDeviceToDevice, s));
     for (int m = 0; m < N; ++m) {
-        MMCSF_HIP(hipMemsetAsync(st.sq.p, 0, R * sizeof(double), s));
+        clear_norms();
         hipLaunchKernelGGL(k_colnorm, dim3(colnorm_grid(f.dims[m], R)), dim3(256), 0, s, U[m],
-                           (uint32_t)f.dims[m], R, 0, st.sq.p, st.mx.p);
-        hipLaunchKernelGGL(k_lambda, dim3(1), dim3(std::max(R, 64)), 0, s, R, 0, st.sq.p, st.mx.p, st.lam.p);
+                           (uint32_t)f.dims[m], R, 0, st.sq, st.mx);
+        hipLaunchKernelGGL(k_lambda, dim3(1), dim3(std::max(R, 64)), 0, s, R, 0, st.sq, st.mx, st.lam);
         hipLaunchKernelGGL(k_scale_cols, dim3(grid_for(f.dims[m] * R, 8192)), dim3(256), 0, s, U[m],
-                           (uint32_t)f.dims[m], R, st.lam.p, st.lamFinal.p);
+                           (uint32_t)f.dims[m], R, st.lam, st.lamFinal);
     }
+    MMCSF_HIP(hipGetLastError());
     if (lambdaHost)
-        MMCSF_HIP(hipMemcpyAsync(lambdaHost, st.lamFinal.p, R * sizeof(double), hipMemcpyDeviceToHost, s));
+        MMCSF_HIP(hipMemcpyAsync(lambdaHost, st.lamFinal, R * sizeof(double), hipMemcpyDeviceToHost, s));
     MMCSF_HIP(hipStreamSynchronize(s));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     if (stats) {
         stats->iterations = it;
         stats->fit = fit;
@@ -534,6 +573,55 @@ void cpd_als(const Format &f, int R, int iters, double tol, int dtype, float *co
         stats->ms_mttkrp = mttkrpMs;
         stats->lu_fallbacks = lu;
     }
+}
+
+void cpd_als(const Format &f, int R, int iters, double tol, int dtype, float *const *U, double *lambdaHost,
+             mmcsf_cpd_stats *stats, hipStream_t s) {
+    DevArray<char> ws(cpd_space(f, R, dtype, nullptr, nullptr));
+    cpd_als_on(f, R, iters, tol, dtype, U, lambdaHost, stats, s, ws.p, true);
+}
+
+// ---------------------------------------------------------------------------
+// CP-ALS without allocation: the work space is this file-scope device array,
+// one per GPU, held until the process ends.  Host code never passes the
+// array's name to a kernel (in host code it only names the symbol): the
+// device address of each GPU's copy is looked up once per device with
+// hipGetSymbolAddress and kept in kScratchAddr.  Every call on one GPU uses
+// the same memory, so calls that overlap on one GPU race.
+// ---------------------------------------------------------------------------
+constexpr size_t kCpdScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) char scratch[kCpdScratchBytes];
+
+constexpr int kMaxDevices = 64;
+static void *kScratchAddr[kMaxDevices];
+static std::mutex kScratchLock;
+
+static char *scratch_on_current_device() {
+    int dev = 0;
+    MMCSF_HIP(hipGetDevice(&dev));
+    MMCSF_REQUIRE(dev >= 0 && dev < kMaxDevices, MMCSF_ERR_UNSUPPORTED, "device index above 63");
+    std::lock_guard<std::mutex> g(kScratchLock);
+    if (!kScratchAddr[dev]) {
+        void *p = nullptr;
+        MMCSF_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        MMCSF_REQUIRE(p != nullptr, MMCSF_ERR_HIP, "no device address for the CP-ALS scratch array");
+        kScratchAddr[dev] = p;
+    }
+    return static_cast<char *>(kScratchAddr[dev]);
+}
+
+size_t cpd_scratch_limit() { return kCpdScratchBytes; }
+
+void cpd_als_noalloc(const Format &f, int R, int iters, double tol, int dtype, float *const *U, double *lambdaHost,
+                     mmcsf_cpd_stats *stats, hipStream_t s) {
+    const size_t need = cpd_space(f, R, dtype, nullptr, nullptr);
+    MMCSF_REQUIRE(need <= kCpdScratchBytes, MMCSF_ERR_UNSUPPORTED,
+                  "CP-ALS work space of " + std::to_string(need) + " bytes exceeds the " +
+                      std::to_string(kCpdScratchBytes) +
+                      "-byte scratch of the allocation-free entry point (largest mode rows x R x 4 bytes, plus "
+                      "the Grams and, for bfloat16 MTTKRP, the bfloat16 factor copies)");
+    char *ws = scratch_on_current_device();
+    cpd_als_on(f, R, iters, tol, dtype, U, lambdaHost, stats, s, ws, false);
 }
 
 }  // namespace mmcsf

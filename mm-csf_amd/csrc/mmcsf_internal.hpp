@@ -224,6 +224,10 @@ void cp_zero(double *loss, hipStream_t s);
 void gram(const float *U, uint64_t rows, int R, double *G, hipStream_t s);
 void cpd_als(const Format &f, int R, int iters, double tol, int dtype, float *const *U, double *lambdaHost,
              mmcsf_cpd_stats *stats, hipStream_t s);
+// the same loop on the library's fixed device scratch (no allocation)
+void cpd_als_noalloc(const Format &f, int R, int iters, double tol, int dtype, float *const *U, double *lambdaHost,
+                     mmcsf_cpd_stats *stats, hipStream_t s);
+size_t cpd_scratch_limit();
 
 // generator (generate.hip)
 void generate_tensor(int ndims, const uint64_t *dims, uint64_t nnz, const int *law,

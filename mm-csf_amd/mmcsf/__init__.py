@@ -108,6 +108,7 @@ def lib():
                                        C.POINTER(C.c_float)]
     L.mmcsf_cpd_als.argtypes = [VP, C.c_int, C.c_int, C.c_double, C.c_int, C.POINTER(VP),
                                 C.POINTER(C.c_double), C.POINTER(CpdStats), VP]
+    L.mmcsf_cpd_als_noalloc.argtypes = L.mmcsf_cpd_als.argtypes
     L.mmcsf_gram.argtypes = [VP, C.c_uint64, C.c_int, VP, VP]
     L.mmcsf_mttkrp_coo_host.argtypes = [VP, C.c_int, C.c_int, C.POINTER(C.POINTER(C.c_float)),
                                         C.POINTER(C.c_float)]
@@ -415,19 +416,22 @@ def gram(U, stream=None):
     return G
 
 
-def cpd_als(fmt, factors, iters=10, tol=1e-5, mttkrp_bf16=False, stream=None):
+def cpd_als(fmt, factors, iters=10, tol=1e-5, mttkrp_bf16=False, stream=None, no_alloc=False):
     """CP-ALS (cpd(), cpd_cpu.cu:484-691) on an MM-CSF format.
 
     ``factors``: device float32 tensors (dims[m], R), updated in place to the
-    final normalised factors.  Returns (lambda, stats dict)."""
+    final normalised factors.  Returns (lambda, stats dict).  ``no_alloc``
+    runs mmcsf_cpd_als_noalloc, whose work space is the library's fixed
+    device scratch (MMCSF_CPD_NOALLOC_SCRATCH_BYTES per GPU)."""
     info = fmt.info()
     R = factors[0].shape[1]
     _check_factors(info["dims"], factors, R)
     fp = (VP * len(factors))(*[VP(f.data_ptr()) for f in factors])
     lam = (C.c_double * R)()
     st = CpdStats()
-    _check(lib().mmcsf_cpd_als(fmt._h, int(R), int(iters), float(tol), BF16 if mttkrp_bf16 else F32, fp, lam,
-                               C.byref(st), _stream_ptr(stream)))
+    fn = lib().mmcsf_cpd_als_noalloc if no_alloc else lib().mmcsf_cpd_als
+    _check(fn(fmt._h, int(R), int(iters), float(tol), BF16 if mttkrp_bf16 else F32, fp, lam, C.byref(st),
+              _stream_ptr(stream)))
     n = st.iterations
     return np.array(lam[:], dtype=np.float64), {
         "iterations": n, "fit": st.fit, "fit_history": list(st.fit_history[: min(n, 64)]),
