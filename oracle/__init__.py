@@ -71,6 +71,10 @@ def lib():
         L.orc_cp_resid_f64_mt.argtypes = [C.c_int, C.c_uint64, U32PP, F32P, C.c_uint64, C.POINTER(D64P),
                                           D64P, D64P, C.c_int]
         L.orc_cp_resid_f64_mt.restype = C.c_int
+        L.orc_mm_partition.argtypes = [C.c_int, C.c_uint64, C.POINTER(U64P), U64P, C.c_int, C.c_int, C.c_int,
+                                       C.POINTER(C.POINTER(C.c_int)), C.POINTER(U64P), C.POINTER(U64P),
+                                       C.POINTER(U64P), U64P]
+        L.orc_mm_partition.restype = C.c_int
         _LIB = L
     return _LIB
 
@@ -354,6 +358,89 @@ def build_format(fmt_type, dims, inds, vals, mode=0, fbr_threshold=99999999):
                 "partPerNnz": np.full(nnz, 2, dtype=np.uint64),
                 "redunMode": redundant_mode(dims)}
     raise ValueError(f"unsupported format type {fmt_type}")
+
+
+def dense_slice_ptr(ndims, dims_slice, fbrPtr, fbrIdx):
+    """denseSlcPtr of util.h:2203-2209: the first level-1 entry of each slice
+    coordinate; the entry after the last slice holds that slice's START (the
+    reference writes fbrPtr[0][slc] there, not fbrPtr[0][slc + 1])."""
+    sp = np.zeros(dims_slice + 1, dtype=np.uint64)
+    p0 = np.asarray(fbrPtr[0], dtype=np.uint64)
+    i0 = np.asarray(fbrIdx[0], dtype=np.int64)
+    ns = len(i0)
+    if ns:
+        sp[i0] = p0[:ns]
+        ends = p0[1:ns + 1].copy()
+        ends[-1] = p0[ns - 1]
+        sp[i0 + 1] = ends
+    return sp
+
+
+def nnz_per_fiber(ndims, fbrPtr):
+    """nnzPerFiber of util.h:2211-2224: nonzeros under each level-1 entry
+    (fbrPtr[1].size() entries, the last one 0)."""
+    p1 = np.asarray(fbrPtr[1], dtype=np.int64)
+    if ndims == 3:
+        c = np.diff(p1)
+    else:
+        p2 = np.asarray(fbrPtr[2], dtype=np.int64)
+        c = p2[p1[1:]] - p2[p1[:-1]]
+    return np.concatenate([c, [0]]).astype(np.uint64)
+
+
+def mm_partition(dims, inds, vals, fb_th=1, load_mode=0, fbr_threshold=99999999):
+    """The MM-CSF mixed-mode partition (mm_partition_reuseBased,
+    util.h:2167-2512, without the override of 2517) and its partitions
+    (populate_paritions, sort_MI_CSF, create_TiledHCSR, create_fbrLikeSlcInds;
+    mttkrp.cu:306-395).  The per-nonzero loop runs sequentially in the order
+    the reference walks the tensor: sorted by every non-redundant mode's order
+    in turn (the arrays are shared, util.h:380-381), i.e. by the last one's,
+    with the canonical full-key order inside equal keys.
+
+    Returns dict: order (tensor positions in loop order), part (per loop
+    position), partitions[p] = None or a dict like build_format(12)."""
+    N = len(dims)
+    nnz = len(vals)
+    vals = np.asarray(vals, dtype=np.float32)
+    orders = [order_tensormode(dims, m) for m in range(N)]
+    red = redundant_mode(dims)
+    red = -1 if red is None else red
+    sp, f1, cnt = [None] * N, [None] * N, [None] * N
+    last = None
+    for m in range(N):
+        if m == red:
+            continue
+        perm, keys = _sorted_keys(inds, orders[m])
+        ptr, idx, nf = build_csf(keys, MMCSF_CREATE_HCSR_THRESHOLD, NO_SPLIT)
+        sp[m] = dense_slice_ptr(N, dims[orders[m][0]], ptr, idx)
+        f1[m] = np.ascontiguousarray(idx[1], dtype=np.uint64)
+        cnt[m] = nnz_per_fiber(N, ptr)
+        last = perm
+    xi = [np.ascontiguousarray(np.asarray(a)[last], dtype=np.uint64) for a in inds]
+    dummy = np.zeros(1, dtype=np.uint64)
+    sp_p = (U64P * N)(*[(a if a is not None else dummy).ctypes.data_as(U64P) for a in sp])
+    f1_p = (U64P * N)(*[(a if a is not None else dummy).ctypes.data_as(U64P) for a in f1])
+    cnt_p = (U64P * N)(*[(a if a is not None else dummy).ctypes.data_as(U64P) for a in cnt])
+    oarr = [(C.c_int * N)(*o) for o in orders]
+    o_p = (C.POINTER(C.c_int) * N)(*[C.cast(o, C.POINTER(C.c_int)) for o in oarr])
+    part = np.zeros(nnz, dtype=np.uint64)
+    d = _u64(dims)
+    lib().orc_mm_partition(N, nnz, _parr(xi, U64P), _p64(d), red, int(fb_th), int(load_mode), o_p, sp_p,
+                           f1_p, cnt_p, _p64(part))
+    parts = []
+    for p in range(N):
+        sel = np.nonzero(part == p)[0]
+        if len(sel) == 0:
+            parts.append(None)
+            continue
+        pin = [np.asarray(a)[sel] for a in xi]
+        perm, keys = _sorted_keys(pin, orders[p])
+        fs = FBRS_THRESHOLD_MMCSF if N == 4 else NO_SPLIT
+        ptr, idx, nf = build_csf(keys, fbr_threshold, fs)
+        parts.append({"modeOrder": orders[p], "keys": keys, "vals": vals[last][sel][perm], "fbrPtr": ptr,
+                      "fbrIdx": idx, "nFibers": nf, "fbrLikeSlcInds": fbr_like_slc_inds(ptr, idx, nf),
+                      "nnz": len(sel)})
+    return {"order": np.asarray(last, dtype=np.int64), "part": part, "partitions": parts, "redunMode": red}
 
 
 def redundant_mode(dims):

@@ -603,3 +603,111 @@ int orc_cp_resid_f64_mt(int ndims, uint64_t nnz, const uint32_t *const *inds,
     *loss = 0.5 * sq;
     return rc;
 }
+
+/* ------------------------------------------------------------------------ */
+/* MM-CSF mixed-mode partition: mm_partition_reuseBased, util.h:2167-2512,  */
+/* with the heuristic's own choice instead of the override `mode = 2` of    */
+/* util.h:2517, executed sequentially in index order (the reference runs the */
+/* loop under OpenMP with unsynchronised decrements, util.h:2500-2512).     */
+/*                                                                          */
+/* inds: the tensor in the order the reference's loop walks it (after the   */
+/* sorts of every non-redundant mode, the last one's order).  For every      */
+/* mode m != redunMode: sp[m] (dims[order_m[0]] + 1 entries) = denseSlcPtr   */
+/* (util.h:2203-2209, including its end pointer of the last slice, which is  */
+/* the slice's start), f1[m] = fbrIdx[1] (level-1 coordinates), cnt[m] =     */
+/* nnzPerFiber (counts per level-1 entry; decremented here, util.h:2500),    */
+/* order[m] = modeOrder of arrX[m].  fbTh = -h.  Out: part[x].               */
+/* ------------------------------------------------------------------------ */
+static int64_t orc_bsearch1(const uint64_t *arr, int64_t left, int64_t right, uint64_t value)
+{
+    /* binarySearch1, util.h:2128-2141 (inclusive right bound) */
+    while (left <= right) {
+        int64_t middle = (int64_t)(((uint32_t)left + (uint32_t)right) >> 1);
+        if (arr[middle] == value) return middle;
+        else if (arr[middle] > value) right = middle - 1;
+        else left = middle + 1;
+    }
+    return -1;
+}
+
+int orc_mm_partition(int ndims, uint64_t nnz, const uint64_t *const *inds, const uint64_t *dims,
+                     int redunMode, int fbTh, int loadMode, const int *const *order, const uint64_t *const *sp,
+                     const uint64_t *const *f1, uint64_t *const *cnt, uint64_t *part)
+{
+    int N = ndims;
+    /* same-fiber pairs, shortMode / longMode (util.h:2261-2310): the last
+     * detected pair wins */
+    int same[4][4] = {{0}};
+    int shortMode = 0, longMode = -1;
+    for (int m = 1; m < N; ++m) {
+        for (int q = 0; q < m; ++q) {
+            if (order[m][1] == order[q][0] && order[m][0] == order[q][1]) {
+                same[q][m] = 1;
+                shortMode = dims[q] <= dims[m] ? q : m;
+                longMode = dims[q] <= dims[m] ? m : q;
+                break;
+            }
+        }
+    }
+    /* the reference tests m-2 before m-1 (m = 2) and m-3, m-2, m-1 (m = 3):
+     * the loop above scans q ascending and stops at the first match, which
+     * is that order */
+    /* shortestMode (util.h:2253): X.modeOrder of the loaded tensor is
+     * order_tensormode(X, -m) (load_tensor, util.h:367) */
+    int o0[ORC_MAXD];
+    orc_order_tensormode(N, dims, loadMode, 0, o0);
+    int shortestMode = dims[o0[0]] <= dims[o0[1]] ? o0[0] : o0[1];
+    uint64_t fbrNnz[ORC_MAXD], fbrNo[ORC_MAXD];
+    int copied[ORC_MAXD];
+    for (uint64_t x = 0; x < nnz; ++x) {
+        for (int m = 0; m < N; ++m) {
+            int copyFrom = -1;
+            copied[m] = 0;
+            for (int q = 0; q < m; ++q)
+                if (same[q][m]) { copyFrom = q; break; }
+            if (copyFrom >= 0) {
+                /* util.h:2372-2383: the second mode of a pair copies the first */
+                fbrNnz[m] = fbrNnz[copyFrom];
+                fbrNo[m] = 99999;
+                copied[m] = 1;
+            } else {
+                uint64_t slc = inds[m][x];
+                uint64_t j = inds[order[m][1]][x];
+                int64_t n = (int64_t)(sp[m][slc + 1] - sp[m][slc]);
+                uint64_t fbr = sp[m][slc];
+                int64_t r = orc_bsearch1(f1[m] + fbr, 0, n, j);
+                uint64_t at = (uint64_t)((int64_t)fbr + r);
+                fbrNo[m] = at;
+                fbrNnz[m] = cnt[m][at];
+            }
+        }
+        int modeDone = 0, mode = 0;
+        uint64_t th = (uint64_t)(int64_t)fbTh;
+        for (int m = 0; m < N && !modeDone; ++m) {
+            uint64_t mx = 0;
+            for (int q = 0; q < N; ++q)
+                if (q != m && fbrNnz[q] > mx) mx = fbrNnz[q];
+            if (fbrNnz[m] >= th * mx) {
+                modeDone = 1;
+                int pairAbove = 0;
+                for (int q = m + 1; q < N; ++q) pairAbove |= same[m][q];
+                /* util.h:2399-2435: a mode starting a same-fiber pair takes shortMode */
+                mode = pairAbove ? shortMode : m;
+            }
+        }
+        if (!modeDone) mode = shortestMode;
+        /* util.h:2493-2512; a mode whose count was copied from its pair has
+         * no fiber of its own (fbrNo 99999): no decrement (the reference's
+         * write there is out of range unless the mode is skipped anyway) */
+        for (int i = 0; i < N; ++i) {
+            if (copied[i]) continue;
+            if (mode == shortMode) {
+                if (i != shortMode && i != longMode) cnt[i][fbrNo[i]]--;
+            } else {
+                if (i != redunMode && i != mode) cnt[i][fbrNo[i]]--;
+            }
+        }
+        part[x] = (uint64_t)mode;
+    }
+    return 0;
+}

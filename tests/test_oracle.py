@@ -593,3 +593,81 @@ def test_threaded_cp_residuals_match_numpy():
     v = D.cp_model_values(inds, A).astype(np.float32)   # exact in float32 (dyadic)
     L, e = O.cp_resid_f64_mt(inds, v, A)
     assert L == 0.0 and not e.any()
+
+
+# ---------------------------------------------------------------------------
+# MM-CSF mixed-mode partition (mm_partition_reuseBased without the override),
+# pinned by hand-derived cases
+# ---------------------------------------------------------------------------
+def test_mm_partition_toy_hand_derived(golden_dir):
+    """toy.tns, -h 1: orders [0,1,2], [1,0,2], [2,0,1]; mode 1 is redundant
+    (its fibers copy mode 0's), so a nonzero goes to mode 0 (shortMode) when
+    its (i,j) fiber holds at least as many nonzeros as its (k,i) fiber, else
+    to mode 2.  Walk order (k,i,j): tensor positions 0,2,4,1,5,7,3,6.  The
+    sixth and eighth nonzeros sit in the last slice of a mode's CSF, whose
+    denseSlcPtr end equals its start: their fibers resolve to the entry
+    before that slice (counts 2 and 0)."""
+    dims, inds, vals = O.load_tns(f"{golden_dir}/toy.tns")
+    r = O.mm_partition(dims, inds, vals)
+    assert r["redunMode"] == 1
+    np.testing.assert_array_equal(r["order"], [0, 2, 4, 1, 5, 7, 3, 6])
+    np.testing.assert_array_equal(r["part"], [2, 2, 0, 0, 0, 0, 0, 0])
+    p0, p1, p2 = r["partitions"]
+    assert p1 is None
+    assert p0["modeOrder"] == [0, 1, 2] and p0["nnz"] == 6
+    np.testing.assert_array_equal(p0["fbrPtr"][0], [0, 1, 3, 5])
+    np.testing.assert_array_equal(p0["fbrIdx"][0], [0, 1, 2])
+    np.testing.assert_array_equal(p0["fbrPtr"][1], [0, 1, 2, 4, 5, 6])
+    np.testing.assert_array_equal(p0["fbrIdx"][1][:-1], [1, 0, 2, 0, 1])
+    np.testing.assert_array_equal(p0["keys"][2], [1, 2, 0, 1, 2, 1])
+    assert p2["modeOrder"] == [2, 0, 1] and p2["nnz"] == 2
+    np.testing.assert_array_equal(p2["fbrPtr"][0], [0, 1])
+    np.testing.assert_array_equal(p2["fbrPtr"][1], [0, 2])
+    np.testing.assert_array_equal(p2["keys"][2], [0, 2])
+    np.testing.assert_array_equal(p2["vals"], [1, 10])
+
+
+def test_mm_partition_decrement_changes_a_later_choice():
+    """3x3x3, nonzeros A=(0,0,0), B=(0,0,1), C=(0,1,0), G=(2,2,2).  Walk
+    order A, C, B, G.  A: fibers (i,j)=(0,0) of 2 and (k,i)=(0,0) of 2 ->
+    mode 0, and the (k,i) count drops to 1.  C: (0,1) of 1 against that 1 ->
+    mode 0 (with the original count 2 it would be mode 2).  B and G: mode 0."""
+    dims = [3, 3, 3]
+    inds = [np.array([0, 0, 0, 2]), np.array([0, 0, 1, 2]), np.array([0, 1, 0, 2])]
+    vals = np.ones(4, np.float32)
+    r = O.mm_partition(dims, inds, vals)
+    np.testing.assert_array_equal(r["order"], [0, 2, 1, 3])
+    np.testing.assert_array_equal(r["part"], [0, 0, 0, 0])
+
+
+def test_mm_partition_long_fiber_of_the_last_mode_wins():
+    """3x3x3, nonzeros (0,j,0) for j=0..2 and G=(2,2,2): each (i,j) fiber holds
+    one nonzero, the (k,i)=(0,0) fiber three -> mode 2 for all three (choosing
+    mode 2 decrements only mode 0's counts); G -> mode 0."""
+    dims = [3, 3, 3]
+    inds = [np.array([0, 0, 0, 2]), np.array([0, 1, 2, 2]), np.array([0, 0, 0, 2])]
+    r = O.mm_partition(dims, inds, np.ones(4, np.float32))
+    np.testing.assert_array_equal(r["order"], [0, 1, 2, 3])
+    np.testing.assert_array_equal(r["part"], [2, 2, 2, 0])
+    assert r["partitions"][2]["nnz"] == 3 and r["partitions"][0]["nnz"] == 1
+
+
+@pytest.mark.parametrize("dims,skew,fbth", [([60, 40, 30], [1.3, 1.2, None], 1), ([60, 40, 30], None, 2),
+                                             ([30, 20, 25, 12], [1.2, None, 1.1, None], 1)])
+def test_mm_partition_partitions_cover_the_tensor(dims, skew, fbth):
+    rng = np.random.default_rng(len(dims) + fbth)
+    inds, vals = random_tensor(rng, dims, 4000, skew)
+    r = O.mm_partition(dims, inds, vals, fb_th=fbth, fbr_threshold=8)
+    N = len(dims)
+    assert set(r["part"].tolist()) <= set(range(N))
+    got = []
+    for p, pp in enumerate(r["partitions"]):
+        if pp is None:
+            assert not (r["part"] == p).any()
+            continue
+        order = pp["modeOrder"]
+        assert pp["nnz"] == int((r["part"] == p).sum())
+        coords = np.stack([pp["keys"][order.index(m)] for m in range(N)], 1)
+        got += [tuple(c) + (v,) for c, v in zip(coords.tolist(), pp["vals"].tolist())]
+    want = [tuple(int(a[x]) for a in inds) + (float(vals[x]),) for x in range(len(vals))]
+    assert sorted(got) == sorted(want)
