@@ -73,6 +73,10 @@ typedef struct mmcsf_options {
     int modeTrees;            /* -a 1|0 (new flag, 1): MM-CSF also keeps a root tree per
                                  other mode, so every mode's MTTKRP is a root traversal;
                                  0 = single tree, non-root modes scatter (the reference) */
+    int mixedPart;            /* -M 1|0 (new flag, 0): MM-CSF partitions the nonzeros by the
+                                 reference's heuristic (mm_partition_reuseBased,
+                                 util.h:2386-2512, threshold -h) instead of its override
+                                 to partition 2 (util.h:2517); no extra mode trees */
     char inFileName[4096];    /* -i */
     char outFileName[4096];   /* -o */
     char m0[16], m1[16], m2[16];  /* -p/-q/-r, unused like the reference */
@@ -190,6 +194,8 @@ typedef struct mmcsf_format_info {
     uint64_t deviceBytes;      /* device memory held by the format (every tree, its
                                   pointer / index / value arrays, start bitmasks, chunk
                                   costs, bins and hot-row tables) */
+    uint64_t partNnz[4];       /* MM-CSF: nonzeros per partition (X.totnnzPerPart) */
+    int mixedPart;             /* MM-CSF built with -M 1 */
 } mmcsf_format_info;
 
 int mmcsf_format_info_get(const mmcsf_format *f, mmcsf_format_info *info);
@@ -206,6 +212,11 @@ int mmcsf_format_info_get(const mmcsf_format *f, mmcsf_format_info *info);
  *   HB-CSF: "COOinds" k (tensor mode k), "COOvals", "CSLslicePtr",
  *   "CSLsliceIdx", "CSLinds" k (k = mode1 / mode2 of modeOrder),
  *   "CSLvals", "CSLslcMapperBin" k.
+ *   MM-CSF built with -M 1 (k = 8 * partition + level / mode):
+ *   "partPerNnz" (the partition of every nonzero, in the order the
+ *   reference's partition loop walks the tensor), "partOrder" (that order as
+ *   positions of the tensor the format was built from), "P.modeOrder",
+ *   "P.fbrPtr", "P.fbrIdx", "P.inds", "P.vals", "P.fbrLikeSlcInds".
  * *len receives the element count; nothing is written when cap < *len.
  */
 int mmcsf_format_export(const mmcsf_format *f, const char *name, int k,

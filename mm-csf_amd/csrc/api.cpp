@@ -249,6 +249,132 @@ int redundant_mode(int ndims, const uint64_t *dims) {
     return red;
 }
 
+// MM-CSF with the mixed-mode partition: mm_partition_reuseBased
+// (util.h:2167-2512) with the heuristic's own choice instead of the override
+// to partition 2 (util.h:2517), then populate_paritions, sort_MI_CSF,
+// create_TiledHCSR and create_fbrLikeSlcInds per partition
+// (mttkrp.cu:306-395).  The level-1 entry of every nonzero in every mode's
+// CSF comes from GPU sort / scan passes; the decisions then run on the host
+// in the order the reference's loop walks the tensor, because each one
+// reads the fiber counts that every earlier decision decremented.
+void build_mixed(const Tensor &t, const mmcsf_options *opt, bool nat, Format &f, hipStream_t s) {
+    const int N = t.ndims;
+    const uint64_t n = t.nnz;
+    const uint64_t kNoSplit = 999999999ull;
+    int orders[kMaxDims][kMaxDims];
+    for (int m = 0; m < N; ++m) order_tensormode(N, t.dims, m, nat, orders[m]);
+    const int red = f.redunMode;
+    // same-fiber pairs (util.h:2261-2310): mode m pairs with the first q < m
+    // whose (order[0], order[1]) it swaps; the last pair sets shortMode / longMode
+    bool same[kMaxDims][kMaxDims] = {};
+    int shortMode = 0, longMode = -1;
+    for (int m = 1; m < N; ++m)
+        for (int q = 0; q < m; ++q)
+            if (orders[m][1] == orders[q][0] && orders[m][0] == orders[q][1]) {
+                same[q][m] = true;
+                shortMode = t.dims[q] <= t.dims[m] ? q : m;
+                longMode = t.dims[q] <= t.dims[m] ? m : q;
+                break;
+            }
+    // shortestMode (util.h:2253) from the loaded tensor's order, which
+    // load_tensor takes for -m (util.h:367)
+    int o0[kMaxDims];
+    order_tensormode(N, t.dims, (int)opt->mode, nat, o0);
+    const int shortestMode = t.dims[o0[0]] <= t.dims[o0[1]] ? o0[0] : o0[1];
+    // every non-redundant mode's CSF without the -f split (create_HCSR forces
+    // 99999999 for type 12, util.h:763-764): the level-1 entry of each nonzero
+    // and the nonzeros under each entry.  The arrays of the reference's
+    // tensors are shared (util.h:380-381), so its loop walks the tensor in the
+    // order of the last of these sorts.
+    // nnzPerFiber is ITYPE (64-bit) in the reference: a fiber that the
+    // last-slice search stands in for can be decremented below zero and wraps
+    std::vector<uint32_t> id[kMaxDims], walk;
+    std::vector<uint64_t> cnt[kMaxDims];
+    for (int m = 0; m < N; ++m) {
+        if (m == red) continue;
+        DevArray<uint32_t> keys[kMaxDims], pos, idT, cntD;
+        DevArray<float> vals;
+        sort_coo(t, orders[m], keys, vals, s, &pos);
+        Csf c;
+        build_csf(N, t.dims, n, orders[m], keys, std::move(vals), 99999999ull, kNoSplit, c, s);
+        level1_ids(c, pos.p, idT, cntD, s);
+        id[m] = to_host(idT, n);
+        const std::vector<uint32_t> c32 = to_host(cntD, c.cnt[1] + 1);
+        cnt[m].assign(c32.begin(), c32.end());
+        walk = to_host(pos, n);
+    }
+    // the decisions, in walk order (util.h:2330-2512)
+    const uint64_t th = (uint64_t)(int64_t)(int)opt->MIfbTh;  // int fbTh = Opt.MIfbTh
+    std::vector<uint8_t> partT(n);
+    f.partPerNnz.assign(n, 0);
+    for (uint64_t k = 0; k < n; ++k) {
+        const uint32_t x = walk[k];
+        uint64_t len[kMaxDims];
+        uint32_t fno[kMaxDims] = {0, 0, 0, 0};
+        bool copied[kMaxDims] = {false, false, false, false};
+        for (int m = 0; m < N; ++m) {
+            for (int q = 0; q < m && !copied[m]; ++q)
+                if (same[q][m]) {
+                    len[m] = len[q];  // the second mode of a pair takes the first's count
+                    copied[m] = true;
+                }
+            if (copied[m]) continue;
+            fno[m] = id[m][x];
+            MMCSF_REQUIRE(fno[m] != 0xFFFFFFFFu, MMCSF_ERR_UNSUPPORTED,
+                          "mixed-mode partition: a mode's CSF has one slice, and the reference's search for "
+                          "a nonzero outside its first fiber reads before the fiber array");
+            len[m] = cnt[m][fno[m]];
+        }
+        int mode = -1;
+        for (int m = 0; m < N && mode < 0; ++m) {
+            uint64_t mx = 0;
+            for (int q = 0; q < N; ++q)
+                if (q != m) mx = std::max(mx, len[q]);
+            if (len[m] >= th * mx) {
+                bool pairAbove = false;
+                for (int q = m + 1; q < N; ++q) pairAbove = pairAbove || same[m][q];
+                mode = pairAbove ? shortMode : m;
+            }
+        }
+        if (mode < 0) mode = shortestMode;
+        // the counts of the fibers this nonzero leaves (util.h:2493-2512);
+        // a copied mode has no fiber of its own
+        for (int i = 0; i < N; ++i) {
+            if (copied[i]) continue;
+            const bool dec = mode == shortMode ? (i != shortMode && i != longMode) : (i != red && i != mode);
+            if (dec) cnt[i][fno[i]]--;
+        }
+        partT[x] = (uint8_t)mode;
+        f.partPerNnz[k] = (uint32_t)mode;
+    }
+    for (int m = 0; m < N; ++m) {
+        id[m].clear();
+        id[m].shrink_to_fit();
+    }
+    // the partitions: their nonzeros in the canonical order of their own
+    // mode order, a CSF with the -f split and, 4-D, the fbrS split at 128
+    for (int p = 0; p < N; ++p) {
+        std::vector<uint32_t> sel;
+        for (uint64_t x = 0; x < n; ++x)
+            if (partT[x] == p) sel.push_back((uint32_t)x);
+        f.partNnz[p] = sel.size();
+        if (sel.empty()) continue;
+        DevArray<uint32_t> selD(sel.size());
+        MMCSF_HIP(hipMemcpyAsync(selD.p, sel.data(), sel.size() * 4, hipMemcpyHostToDevice, s));
+        Tensor sub;
+        gather_tensor(t, selD.p, sel.size(), sub, s);
+        DevArray<uint32_t> keys[kMaxDims];
+        DevArray<float> vals;
+        sort_coo(sub, orders[p], keys, vals, s);
+        const uint64_t fs = N == 4 ? 128 : kNoSplit;  // util.h:1325-1326
+        build_csf(N, t.dims, sub.nnz, orders[p], keys, std::move(vals), opt->fbrThreashold, fs, f.part[p], s);
+        for (int l = 1; l < N; ++l) build_hot_table(f.part[p], l, t.dims[orders[p][l]], f.part[p].hot[l], s);
+        MMCSF_HIP(hipStreamSynchronize(s));
+    }
+    f.partOrder = std::move(walk);
+    f.mixed = true;
+}
+
 }  // namespace
 }  // namespace mmcsf
 
@@ -273,6 +399,7 @@ uint64_t format_bytes(const mmcsf::Format &f) {
     for (int l = 0; l < mmcsf::kMaxDims; ++l) {
         b += bytes_of(f.cooKeys[l]) + bytes_of(f.hyb.cooInd[l]);
         if (f.hasTree[l]) b += csf_bytes(f.tree[l]);
+        if (f.mixed) b += csf_bytes(f.part[l]);
     }
     return b;
 }
@@ -306,6 +433,7 @@ int mmcsf_options_default(mmcsf_options *o) {
     o->factorDtype = MMCSF_F32;
     o->seed = 1;
     o->modeTrees = 1;
+    o->mixedPart = 0;
     std::strcpy(o->m0, "012");
     std::strcpy(o->m1, "120");
     std::strcpy(o->m2, "201");
@@ -340,6 +468,7 @@ int mmcsf_options_parse(int argc, char **argv, mmcsf_options *o) {
                 case 'd': o->factorDtype = std::atoi(v) == 1 ? MMCSF_BF16 : MMCSF_F32; break;
                 case 'n': o->cpdIters = as_u64(v); break;
                 case 'a': o->modeTrees = std::atoi(v) != 0; break;
+                case 'M': o->mixedPart = std::atoi(v) != 0; break;
                 case 'x': o->seed = std::strtoull(v, nullptr, 0); break;
                 case 'i':
                     MMCSF_REQUIRE(std::strlen(v) < sizeof(o->inFileName), MMCSF_ERR_ARG, "input path too long");
@@ -590,7 +719,9 @@ int mmcsf_format_build(const mmcsf_tensor *th, int type, const mmcsf_options *op
         DevArray<uint32_t> keys[kMaxDims];
         DevArray<float> vals;
         const uint64_t kNoSplit = 999999999ull;  // util.h:1323
-        if (type == MMCSF_TYPE_MMCSF) {
+        if (type == MMCSF_TYPE_MMCSF && opt->mixedPart) {
+            build_mixed(t, opt, nat, f, s);
+        } else if (type == MMCSF_TYPE_MMCSF) {
             // every nonzero lands in partition 2 (util.h:2517), whose mode
             // order is order_tensormode(X, 2) (init_tensor, util.h:372-387)
             order_tensormode(t.ndims, t.dims, 2, nat, order);
@@ -689,6 +820,11 @@ int mmcsf_format_info_get(const mmcsf_format *fh, mmcsf_format_info *info) {
             info->COOnnz = f.hyb.cooNnz;
             info->CSLnnz = f.hyb.cslNnz;
             info->HCSRnnz = f.hyb.csf.nnz;
+        } else if (f.mixed) {
+            int big = 0;
+            for (int p = 0; p < f.ndims; ++p)
+                if (f.partNnz[p] > f.partNnz[big]) big = p;
+            c = &f.part[big];
         } else {
             c = &f.csf;
         }
@@ -709,6 +845,11 @@ int mmcsf_format_info_get(const mmcsf_format *fh, mmcsf_format_info *info) {
                 for (int l = 0; l < f.ndims; ++l) info->modeLevel[m][l] = t.cnt[l];
             }
         info->deviceBytes = format_bytes(f);
+        for (int p = 0; p < f.ndims; ++p) info->partNnz[p] = f.mixed ? f.partNnz[p] : (p == 2 ? f.nnz : 0);
+        info->mixedPart = f.mixed ? 1 : 0;
+        if (f.mixed)
+            for (int m = 0; m < f.ndims; ++m)
+                for (int l = 0; l < f.ndims; ++l) info->modeLevel[m][l] = f.part[m].cnt[l];
     });
 }
 
@@ -724,6 +865,8 @@ int mmcsf_format_export(const mmcsf_format *fh, const char *name, int k, void *d
         const bool hyb = f.type == MMCSF_TYPE_HBCSF;
         const bool coo = f.type == MMCSF_TYPE_COO || f.type == MMCSF_TYPE_COO_CPU;
         const Csf &c = hyb ? f.hyb.csf : f.csf;
+        MMCSF_REQUIRE(!f.mixed || nm == "partPerNnz" || nm == "partOrder" || nm.rfind("P.", 0) == 0, MMCSF_ERR_ARG,
+                      "an MM-CSF built with -M 1 exports its partitions (P.<field>, k = 8 * partition + level)");
         auto widen = [&](const std::vector<uint32_t> &v) { u.assign(v.begin(), v.end()); };
         auto need_level = [&](int lvl) {
             MMCSF_REQUIRE(!coo && lvl >= 0 && lvl < N - 1, MMCSF_ERR_ARG, "bad level for " + nm);
@@ -761,7 +904,38 @@ int mmcsf_format_export(const mmcsf_format *fh, const char *name, int k, void *d
             u.assign(all.begin() + off, all.end());
         } else if (nm == "partPerNnz") {
             MMCSF_REQUIRE(f.type == MMCSF_TYPE_MMCSF, MMCSF_ERR_ARG, "partPerNnz exists for MM-CSF");
-            u.assign(f.nnz, 2);
+            if (f.mixed) widen(f.partPerNnz);
+            else u.assign(f.nnz, 2);
+        } else if (nm == "partOrder") {
+            MMCSF_REQUIRE(f.mixed, MMCSF_ERR_ARG, "partOrder exists for MM-CSF built with -M 1");
+            widen(f.partOrder);
+        } else if (nm.rfind("P.", 0) == 0) {
+            MMCSF_REQUIRE(f.mixed, MMCSF_ERR_ARG, nm + " exists for MM-CSF built with -M 1");
+            const int p = k / 8, l = k % 8;
+            MMCSF_REQUIRE(k >= 0 && p < N && f.partNnz[p] > 0, MMCSF_ERR_ARG, "no such partition");
+            const Csf &pc = f.part[p];
+            const std::string fld = nm.substr(2);
+            if (fld == "modeOrder") {
+                u.assign(pc.order, pc.order + N);
+            } else if (fld == "fbrPtr") {
+                MMCSF_REQUIRE(l < N - 1, MMCSF_ERR_ARG, "bad level");
+                widen(to_host(pc.ptr[l], pc.cnt[l] + 1));
+            } else if (fld == "fbrIdx") {
+                MMCSF_REQUIRE(l < N - 1, MMCSF_ERR_ARG, "bad level");
+                widen(to_host(pc.idx[l], pc.cnt[l]));
+                if (l == N - 2 && pc.cnt[l]) u.push_back(u.back());
+            } else if (fld == "inds") {
+                MMCSF_REQUIRE(l < N, MMCSF_ERR_ARG, "bad mode");
+                widen(expand_level(pc, level_of(pc.order, N, l)));
+            } else if (fld == "vals") {
+                isFloat = true;
+                fl = to_host(pc.vals, pc.nnz);
+            } else if (fld == "fbrLikeSlcInds") {
+                widen(to_host(pc.fbrSlc, pc.cnt[1]));
+                u.push_back(u.empty() ? 0 : u.back());
+            } else {
+                fail(MMCSF_ERR_ARG, "unknown field " + nm);
+            }
         } else if (hyb && nm == "COOinds") {
             MMCSF_REQUIRE(k >= 0 && k < N, MMCSF_ERR_ARG, "bad mode");
             widen(to_host(f.hyb.cooInd[level_of(f.hyb.order, N, k)], f.hyb.cooNnz));
@@ -1004,6 +1178,7 @@ namespace {
 
 // the tree whose root level is `mode` (with tensor-order positions), or null
 const Csf *root_tree(const Format &f, int mode) {
+    if (f.mixed) return nullptr;  // partitions keep no tensor-order positions
     if (f.type == MMCSF_TYPE_MMCSF) {
         if (f.csf.order[0] == mode) return &f.csf;
         if (f.hasTree[mode]) return &f.tree[mode];
@@ -1017,6 +1192,8 @@ const Csf *root_tree(const Format &f, int mode) {
 
 Factors cp_factors(const Format &f, int R, int dtype, const void *const *factors) {
     MMCSF_REQUIRE(factors, MMCSF_ERR_ARG, "null argument");
+    MMCSF_REQUIRE(!f.mixed, MMCSF_ERR_UNSUPPORTED,
+                  "CP completion needs a B-CSF or an MM-CSF without the mixed-mode partition (-M 0)");
     MMCSF_REQUIRE(dtype == MMCSF_F32 || dtype == MMCSF_BF16, MMCSF_ERR_ARG, "dtype must be F32 or BF16");
     MMCSF_REQUIRE(cp_supports(R), MMCSF_ERR_UNSUPPORTED, "CP completion needs R % 4 == 0 and R <= 128");
     const size_t need = dtype == MMCSF_F32 || R % 8 == 0 ? 16 : 8;

@@ -966,4 +966,92 @@ uint64_t hot_footprint(const Tensor &t, int m, double frac, hipStream_t s) {
     return k;
 }
 
+
+// ---------------------------------------------------------------------------
+// MM-CSF mixed-mode partition support (mm_partition_reuseBased,
+// util.h:2167-2512): the level-1 entry of every nonzero in a mode's CSF and
+// the nonzeros under each entry (nnzPerFiber)
+// ---------------------------------------------------------------------------
+// first nonzero of level-1 entry j (j = cnt1: nnz), and a mark at it
+__global__ void k_l1_start(int N, const uint32_t *__restrict__ p1, const uint32_t *__restrict__ p2, uint64_t cnt1,
+                           uint64_t nnz, uint32_t *start, uint32_t *mark) {
+    uint64_t j = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (j > cnt1) return;
+    if (j == cnt1) {
+        start[j] = (uint32_t)nnz;
+        return;
+    }
+    const uint32_t x = N == 3 ? p1[j] : p2[p1[j]];
+    start[j] = x;
+    mark[x] = 1u;
+}
+
+__global__ void k_l1_count(const uint32_t *__restrict__ start, uint64_t cnt1, uint32_t *cnt) {
+    uint64_t j = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (j < cnt1) cnt[j] = start[j + 1] - start[j];
+}
+
+// entry of sorted nonzero k = (marks before k) + mark[k] - 1, written at its
+// tensor position; nonzeros of the last slice (from lastStart on) outside
+// its first entry e0 resolve to e0 - 1 (kNoEntry when e0 = 0)
+__global__ void k_l1_ids(const uint32_t *__restrict__ excl, const uint32_t *__restrict__ mark,
+                         const uint32_t *__restrict__ pos, uint64_t nnz, uint32_t lastStart, uint32_t e0,
+                         uint32_t *idT) {
+    uint64_t k = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (k >= nnz) return;
+    uint32_t id = excl[k] + mark[k] - 1u;
+    if (k >= lastStart && id != e0) id = e0 == 0 ? 0xFFFFFFFFu : e0 - 1u;
+    idT[pos[k]] = id;
+}
+
+void level1_ids(const Csf &c, const uint32_t *pos, DevArray<uint32_t> &idT, DevArray<uint32_t> &cnt, hipStream_t s) {
+    const int N = c.ndims;
+    const uint64_t n = c.nnz, c1 = c.cnt[1];
+    idT.alloc(n);
+    cnt.alloc(c1 + 1);
+    if (n == 0) return;
+    Temp tmp;
+    DevArray<uint32_t> start(c1 + 1), mark(n), excl(n);
+    hipLaunchKernelGGL(k_zero_words, dim3(nblk(n)), dim3(kBS), 0, s, mark.p, n);
+    hipLaunchKernelGGL(k_l1_start, dim3(nblk(c1 + 1)), dim3(kBS), 0, s, N, c.ptr[1].p,
+                       N == 4 ? c.ptr[2].p : (const uint32_t *)nullptr, c1, n, start.p, mark.p);
+    hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(kBS), 0, s, cnt.p + c1, (uint64_t)1);
+    hipLaunchKernelGGL(k_l1_count, dim3(nblk(c1)), dim3(kBS), 0, s, start.p, c1, cnt.p);
+    exclusive_sum(mark.p, excl.p, n, s, tmp);
+    // the last slice's first level-1 entry and its first nonzero
+    uint32_t e0 = 0, lastStart = 0;
+    MMCSF_HIP(hipMemcpyAsync(&e0, c.ptr[0].p + (c.cnt[0] - 1), 4, hipMemcpyDeviceToHost, s));
+    MMCSF_HIP(hipStreamSynchronize(s));
+    MMCSF_HIP(hipMemcpyAsync(&lastStart, start.p + e0, 4, hipMemcpyDeviceToHost, s));
+    MMCSF_HIP(hipStreamSynchronize(s));
+    hipLaunchKernelGGL(k_l1_ids, dim3(nblk(n)), dim3(kBS), 0, s, excl.p, mark.p, pos, n, lastStart, e0, idT.p);
+    MMCSF_HIP(hipGetLastError());
+    MMCSF_HIP(hipStreamSynchronize(s));
+}
+
+__global__ void k_gather_sel_u32(const uint32_t *__restrict__ src, const uint32_t *__restrict__ sel, uint64_t n,
+                                 uint32_t *dst) {
+    uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[sel[i]];
+}
+__global__ void k_gather_sel_f32(const float *__restrict__ src, const uint32_t *__restrict__ sel, uint64_t n,
+                                 float *dst) {
+    uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[sel[i]];
+}
+
+// the nonzeros at tensor positions sel[0..n) as a tensor of their own
+void gather_tensor(const Tensor &t, const uint32_t *sel, uint64_t n, Tensor &out, hipStream_t s) {
+    out.ndims = t.ndims;
+    out.nnz = n;
+    for (int m = 0; m < t.ndims; ++m) {
+        out.dims[m] = t.dims[m];
+        out.inds[m].alloc(n);
+        if (n) hipLaunchKernelGGL(k_gather_sel_u32, dim3(nblk(n)), dim3(kBS), 0, s, t.inds[m].p, sel, n, out.inds[m].p);
+    }
+    out.vals.alloc(n);
+    if (n) hipLaunchKernelGGL(k_gather_sel_f32, dim3(nblk(n)), dim3(kBS), 0, s, t.vals.p, sel, n, out.vals.p);
+    MMCSF_HIP(hipGetLastError());
+}
+
 }  // namespace mmcsf

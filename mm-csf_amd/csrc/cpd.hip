@@ -483,8 +483,16 @@ static void cpd_als_on(const Format &f, int R, int iters, double tol, int dtype,
     }
     hipLaunchKernelGGL(k_zero_d, dim3(1), dim3(256), 0, s, st.scal, (uint64_t)4);
     hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, (uint32_t *)st.status, (uint64_t)1);
-    hipLaunchKernelGGL(k_sumsq, dim3(grid_for(f.nnz, 4096)), dim3(256), 0, s, f.csf.vals.p, (uint64_t)f.nnz,
-                       st.scal);
+    // |X|^2 over every stored value (the partitions of a mixed MM-CSF)
+    if (f.mixed) {
+        for (int p = 0; p < N; ++p)
+            if (f.partNnz[p])
+                hipLaunchKernelGGL(k_sumsq, dim3(grid_for(f.partNnz[p], 4096)), dim3(256), 0, s, f.part[p].vals.p,
+                                   (uint64_t)f.partNnz[p], st.scal);
+    } else {
+        hipLaunchKernelGGL(k_sumsq, dim3(grid_for(f.nnz, 4096)), dim3(256), 0, s, f.csf.vals.p, (uint64_t)f.nnz,
+                           st.scal);
+    }
     hipLaunchKernelGGL(k_fill_d, dim3(1), dim3(256), 0, s, st.lam, R, 1.0);
     Factors F{};
     F.dtype = dtype;

@@ -155,6 +155,14 @@ struct Format {
     // mode's MTTKRP is a root traversal (internal; never exported)
     Csf tree[kMaxDims];
     bool hasTree[kMaxDims] = {false, false, false, false};
+    // type 12 with the mixed-mode partition (-M 1): partition p holds the
+    // nonzeros the heuristic assigns to mode p, in a tree of order
+    // order_tensormode(X, p); csf is then empty
+    bool mixed = false;
+    Csf part[kMaxDims];
+    uint64_t partNnz[kMaxDims] = {0, 0, 0, 0};
+    std::vector<uint32_t> partPerNnz;   // per nonzero in the partition loop's order
+    std::vector<uint32_t> partOrder;    // that order as tensor positions
     // type 10
     Hyb hyb;
 };
@@ -182,6 +190,13 @@ void build_hyb(int ndims, const uint64_t *dims, Csf &&src, Hyb &out, hipStream_t
 void build_hot_table(const Csf &c, int level, uint64_t rows, HotTable &h, hipStream_t s);
 // rows of tensor mode m holding `frac` of the nonzeros, most used first
 uint64_t hot_footprint(const Tensor &t, int m, double frac, hipStream_t s);
+// mixed-mode partition: level-1 entry of every nonzero of an unsplit CSF (at
+// its tensor position, resolved like the reference's search over
+// denseSlcPtr; 0xFFFFFFFF where that search reads before the first entry)
+// and the nonzeros under each level-1 entry (cnt1 + 1 entries, the last 0)
+void level1_ids(const Csf &c, const uint32_t *pos, DevArray<uint32_t> &idT, DevArray<uint32_t> &cnt, hipStream_t s);
+// the nonzeros at tensor positions sel[0..n) as a tensor
+void gather_tensor(const Tensor &t, const uint32_t *sel, uint64_t n, Tensor &out, hipStream_t s);
 
 // ---------------------------------------------------------------------------
 // MTTKRP launchers (mttkrp.hip)
@@ -194,8 +209,9 @@ struct Factors {
 
 // evStart / evEnd (optional) bracket the MTTKRP kernel only, after the
 // output has been zeroed.
+// accumulate: add to `out` (not zeroed here; every row written with atomics)
 void mttkrp_csf(const Csf &c, const uint64_t *dims, int outLevel, const Factors &F, float *out,
-                hipStream_t s, hipEvent_t evStart = nullptr, hipEvent_t evEnd = nullptr);
+                hipStream_t s, hipEvent_t evStart = nullptr, hipEvent_t evEnd = nullptr, bool accumulate = false);
 void mttkrp_coo(const Format &f, const Factors &F, float *out, hipStream_t s);
 // MTTKRP of tensor mode `mode` of a CSF-based format (types 8 and 12),
 // through the mode's root tree when the format has one
@@ -204,7 +220,7 @@ void mttkrp_mode(const Format &f, int mode, const Factors &F, float *out, hipStr
 // root-level MTTKRP over 64-nonzero chunks (walk.hip); needs R % 4 == 0 and
 // the start bitmasks of the tree
 bool walk_supports(const Csf &c, int R);
-void mttkrp_root(const Csf &c, const Factors &F, float *out, hipStream_t s);
+void mttkrp_root(const Csf &c, const Factors &F, float *out, hipStream_t s, bool accumulate = false);
 void mttkrp_hyb(const Hyb &h, const uint64_t *dims, const Factors &F, float *out, hipStream_t s);
 void fill_zero(float *p, uint64_t n, hipStream_t s);
 

@@ -484,9 +484,9 @@ void fill_zero(float *p, uint64_t n, hipStream_t s) {
 }
 
 void mttkrp_csf(const Csf &c, const uint64_t *dims, int outLevel, const Factors &F, float *out,
-                hipStream_t s, hipEvent_t evStart, hipEvent_t evEnd) {
+                hipStream_t s, hipEvent_t evStart, hipEvent_t evEnd, bool accumulate) {
     const int N = c.ndims;
-    fill_zero(out, dims[c.order[outLevel]] * (uint64_t)F.R, s);
+    if (!accumulate) fill_zero(out, dims[c.order[outLevel]] * (uint64_t)F.R, s);
     if (evStart) MMCSF_HIP(hipEventRecord(evStart, s));
     struct EndMark {
         hipEvent_t e;
@@ -520,7 +520,7 @@ void mttkrp_csf(const Csf &c, const uint64_t *dims, int outLevel, const Factors 
     a.R = F.R;
     a.perGroup = per_group_for(c.nnz, g.G);
     if (outLevel == 0 && walk_supports(c, F.R)) {
-        mttkrp_root(c, F, out, s);  // checks the alignment its row loads need
+        mttkrp_root(c, F, out, s, accumulate);  // checks the alignment its row loads need
         return;
     }
     if (outLevel > 0) {
@@ -551,6 +551,21 @@ void mttkrp_csf(const Csf &c, const uint64_t *dims, int outLevel, const Factors 
 
 void mttkrp_mode(const Format &f, int mode, const Factors &F, float *out, hipStream_t s, hipEvent_t evStart,
                  hipEvent_t evEnd) {
+    if (f.mixed) {
+        // every partition adds its nonzeros' contributions, with the kernel of
+        // the level the mode takes in its tree (mttkrp_gpu.cu:3610-3719)
+        fill_zero(out, f.dims[mode] * (uint64_t)F.R, s);
+        if (evStart) MMCSF_HIP(hipEventRecord(evStart, s));
+        for (int p = 0; p < f.ndims; ++p) {
+            if (!f.partNnz[p]) continue;
+            int lvl = 0;
+            for (int l = 0; l < f.ndims; ++l)
+                if (f.part[p].order[l] == mode) lvl = l;
+            mttkrp_csf(f.part[p], f.dims, lvl, F, out, s, nullptr, nullptr, true);
+        }
+        if (evEnd) MMCSF_HIP(hipEventRecord(evEnd, s));
+        return;
+    }
     if (f.hasTree[mode]) {
         mttkrp_csf(f.tree[mode], f.dims, 0, F, out, s, evStart, evEnd);
         return;

@@ -73,13 +73,14 @@ bool walk_supports(const Csf &c, int R) {
            (c.nnz == 0 || (c.fbit.p && c.chunkCost.p && (c.ndims == 3 || c.sbit.p)));
 }
 
-void mttkrp_root(const Csf &c, const Factors &F, float *out, hipStream_t s) {
+void mttkrp_root(const Csf &c, const Factors &F, float *out, hipStream_t s, bool accumulate) {
     if (c.nnz == 0) return;
     MMCSF_REQUIRE(walk_supports(c, F.R), MMCSF_ERR_UNSUPPORTED,
                   "root walk needs R % 4 == 0, start bitmasks and factor tables below 4 GiB");
     check_walk_align(c, F, 0, out);
     WalkArgs a = args_of(c, F);
     a.out = out;
+    a.accum = accumulate ? 1 : 0;
     walk_launch<kMttkrp>(c, F, a, s);
 }
 

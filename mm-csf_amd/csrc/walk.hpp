@@ -57,6 +57,7 @@ struct WalkArgs {
     float *resid;          // residuals in tensor order, or null
     double *sumsq;         // sum of e^2 (accumulated), or null
     float alpha;
+    int accum;             // add every row with atomics (other writers share `out`)
     uint32_t nnz;
     uint32_t cnt[4];
     int R;

@@ -247,7 +247,7 @@ __global__ __launch_bounds__(256, (OP == kMttkrp && N == 3 && NB == 1 && G * VEC
         const int col = b * BW + k * VEC;
         if (row == kNone || col >= R) return;
         float *o = a.out + (size_t)row * R + col;
-        if (store) {
+        if (store && !a.accum) {
 #pragma unroll
             for (int e = 0; e < VEC; e += 4) *reinterpret_cast<float4 *>(o + e) = make_float4(v[e], v[e + 1], v[e + 2], v[e + 3]);
         } else {

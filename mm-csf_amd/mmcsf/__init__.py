@@ -43,7 +43,8 @@ class Options(C.Structure):
         ("TBsize", C.c_uint64), ("MIfbTh", C.c_uint64), ("fiberPerWarp", C.c_uint64),
         ("verbose", C.c_int), ("correctness", C.c_int), ("nBin", C.c_uint64),
         ("cpdIters", C.c_uint64), ("natOrdering", C.c_int), ("fbrThreashold", C.c_uint64),
-        ("factorDtype", C.c_int), ("seed", C.c_uint64), ("modeTrees", C.c_int), ("inFileName", C.c_char * 4096), ("outFileName", C.c_char * 4096),
+        ("factorDtype", C.c_int), ("seed", C.c_uint64), ("modeTrees", C.c_int),
+        ("mixedPart", C.c_int), ("inFileName", C.c_char * 4096), ("outFileName", C.c_char * 4096),
         ("m0", C.c_char * 16), ("m1", C.c_char * 16), ("m2", C.c_char * 16),
     ]
 
@@ -54,7 +55,8 @@ class FormatInfo(C.Structure):
         ("modeOrder", C.c_int * 4), ("nLevel", C.c_uint64 * 4), ("nFibers", C.c_uint64),
         ("droppedSlices", C.c_uint64), ("COOnnz", C.c_uint64), ("CSLnnz", C.c_uint64),
         ("HCSRnnz", C.c_uint64), ("redunMode", C.c_int), ("modeTrees", C.c_int),
-        ("modeLevel", (C.c_uint64 * 4) * 4), ("deviceBytes", C.c_uint64),
+        ("modeLevel", (C.c_uint64 * 4) * 4), ("deviceBytes", C.c_uint64), ("partNnz", C.c_uint64 * 4),
+        ("mixedPart", C.c_int),
     ]
 
 
@@ -262,10 +264,14 @@ class Tensor:
 class Format:
     """Device format of one driver type (-t 1/3/8/10/12)."""
 
-    def __init__(self, tensor, fmt_type, mode=0, fbr_threshold=99999999, stream=None, opts=None, mode_trees=True):
+    def __init__(self, tensor, fmt_type, mode=0, fbr_threshold=99999999, stream=None, opts=None, mode_trees=True,
+                 mixed=False, fb_th=1):
         o = opts if opts is not None else default_options()
         o.mode = int(mode)
         o.modeTrees = 1 if mode_trees else 0
+        if mixed:
+            o.mixedPart = 1
+            o.MIfbTh = int(fb_th)
         o.fbrThreashold = int(fbr_threshold) & 0xFFFFFFFFFFFFFFFF
         h = VP()
         _check(lib().mmcsf_format_build(tensor._h, int(fmt_type), C.byref(o), _stream_ptr(stream),
@@ -283,14 +289,14 @@ class Format:
             "CSLnnz": i.CSLnnz, "HCSRnnz": i.HCSRnnz, "redunMode": i.redunMode,
             "modeTrees": i.modeTrees,
             "modeLevel": [list(i.modeLevel[m][: i.ndims]) for m in range(i.ndims)],
-            "deviceBytes": i.deviceBytes,
+            "deviceBytes": i.deviceBytes, "partNnz": list(i.partNnz[: i.ndims]), "mixedPart": i.mixedPart,
         }
 
     def export(self, name, k=0):
         """A reference member as a numpy array (uint64, or float32 for values)."""
         n = C.c_uint64()
         _check(lib().mmcsf_format_export(self._h, name.encode(), int(k), None, 0, C.byref(n)))
-        is_float = name in ("vals", "COOvals", "CSLvals")
+        is_float = name in ("vals", "COOvals", "CSLvals", "P.vals")
         out = np.zeros(max(n.value, 1), dtype=np.float32 if is_float else np.uint64)
         _check(lib().mmcsf_format_export(self._h, name.encode(), int(k), out.ctypes.data_as(VP),
                                          n.value, C.byref(n)))

@@ -34,6 +34,12 @@
  *                          order), on 3-D and 4-D skewed tensors; toy.tns.
  *   orc_cp_resid_f64_mt    numpy float64 model values (dense.cp_loss) and a
  *                          planted exact model with zero residual.
+ *   orc_mm_partition       hand-derived partitions of toy.tns (including
+ *                          two last-slice searches and a decremented count)
+ *                          and of two 4-nonzero cases: one where a decrement
+ *                          changes a later choice, one where the last mode's
+ *                          long fiber wins; every nonzero lands in exactly
+ *                          one partition.
  */
 #include <stdint.h>
 #include <stdlib.h>
