@@ -77,8 +77,9 @@ typedef struct mmcsf_options {
                                  reference's heuristic (mm_partition_reuseBased,
                                  util.h:2386-2512, threshold -h) instead of its override
                                  to partition 2 (util.h:2517); no extra mode trees */
-    char inFileName[4096];    /* -i */
+    char inFileName[4096];    /* -i: a .tns file or a format cache (mmcsf_format_save) */
     char outFileName[4096];   /* -o */
+    char cacheFileName[4096]; /* -k (new flag): write the built format to this cache file */
     char m0[16], m1[16], m2[16];  /* -p/-q/-r, unused like the reference */
 } mmcsf_options;
 
@@ -223,6 +224,29 @@ int mmcsf_format_export(const mmcsf_format *f, const char *name, int k,
                         void *dst, uint64_t cap, uint64_t *len);
 
 void mmcsf_format_free(mmcsf_format *f);
+
+/*
+ * Binary cache of a built format (no reference counterpart; the reference
+ * rebuilds every format from the .tns text on each run, util.h:313-370 and
+ * mttkrp.cu:104-395).  mmcsf_format_save writes every array of every tree the
+ * format holds (the reference's arrays, the per-mode root trees, the
+ * mixed-mode partitions, start bitmasks, chunk costs, bins, hot-row tables)
+ * after an 8-byte magic "MMCSFFMT", a uint32 version (1) and a byte-order
+ * mark; mmcsf_format_load rebuilds the format from it, equal bit for bit to
+ * the one saved, without reading text, sorting or building.  Errors
+ * (MMCSF_ERR_IO) on a missing file, another version or byte order, a
+ * truncated file or a bad end mark.
+ */
+int mmcsf_format_save(const mmcsf_format *f, const char *path);
+int mmcsf_format_load(const char *path, mmcsf_format **out);
+/* 1 if the file at path starts with the format-cache magic, else 0. */
+int mmcsf_format_is_cache(const char *path);
+/* The nonzeros a format stores, as a new device COO tensor (in the format's
+ * stored order; HB-CSF parts and mixed-mode partitions one after another):
+ * the tensor a cached format was built from, up to the order of nonzeros.
+ * No reference counterpart (the reference keeps its Tensor beside the
+ * formats). */
+int mmcsf_format_to_tensor(const mmcsf_format *f, mmcsf_tensor **out);
 
 /* ---- MTTKRP ------------------------------------------------------------------ */
 

@@ -231,6 +231,47 @@ std::vector<uint32_t> expand_level(const Csf &c, int l) {
     return out;
 }
 
+// the nonzeros a format stores, per tensor mode, in its stored order (HB-CSF
+// parts and mixed-mode partitions one after another)
+void format_nonzeros(const Format &f, std::vector<std::vector<uint32_t>> &inds, std::vector<float> &vals) {
+    const int N = f.ndims;
+    inds.assign(N, {});
+    vals.clear();
+    auto add_csf = [&](const Csf &c) {
+        if (!c.nnz) return;
+        for (int m = 0; m < N; ++m) {
+            std::vector<uint32_t> v = expand_level(c, level_of(c.order, N, m));
+            inds[m].insert(inds[m].end(), v.begin(), v.end());
+        }
+        std::vector<float> v = to_host(c.vals, c.nnz);
+        vals.insert(vals.end(), v.begin(), v.end());
+    };
+    if (f.type == MMCSF_TYPE_COO || f.type == MMCSF_TYPE_COO_CPU) {
+        for (int m = 0; m < N; ++m) inds[m] = to_host(f.cooKeys[level_of(f.cooOrder, N, m)], f.nnz);
+        vals = to_host(f.cooVals, f.nnz);
+    } else if (f.type == MMCSF_TYPE_HBCSF) {
+        const Hyb &h = f.hyb;
+        for (int m = 0; m < N; ++m) inds[m] = to_host(h.cooInd[level_of(h.order, N, m)], h.cooNnz);
+        vals = to_host(h.cooVal, h.cooNnz);
+        if (h.cslNnz) {
+            const std::vector<uint32_t> ptr = to_host(h.cslPtr, h.nCsl + 1), idx = to_host(h.cslIdx, h.nCsl);
+            std::vector<uint32_t> slc(h.cslNnz);
+            for (uint64_t q = 0; q < h.nCsl; ++q)
+                for (uint32_t x = ptr[q]; x < ptr[q + 1]; ++x) slc[x] = idx[q];
+            const std::vector<uint32_t> i1 = to_host(h.cslInd1, h.cslNnz), i2 = to_host(h.cslInd2, h.cslNnz);
+            const std::vector<uint32_t> *lv[3] = {&slc, &i1, &i2};
+            for (int l = 0; l < 3; ++l) inds[h.order[l]].insert(inds[h.order[l]].end(), lv[l]->begin(), lv[l]->end());
+            const std::vector<float> v = to_host(h.cslVal, h.cslNnz);
+            vals.insert(vals.end(), v.begin(), v.end());
+        }
+        add_csf(h.csf);
+    } else if (f.mixed) {
+        for (int p = 0; p < N; ++p) add_csf(f.part[p]);
+    } else {
+        add_csf(f.csf);
+    }
+}
+
 bool mmcsf_type_ok(int type) {
     return type == MMCSF_TYPE_COO_CPU || type == MMCSF_TYPE_COO || type == MMCSF_TYPE_BCSF ||
            type == MMCSF_TYPE_BCSF_ANYMODE || type == MMCSF_TYPE_ONECSF || type == MMCSF_TYPE_HBCSF ||
@@ -478,6 +519,10 @@ int mmcsf_options_parse(int argc, char **argv, mmcsf_options *o) {
                     MMCSF_REQUIRE(std::strlen(v) < sizeof(o->outFileName), MMCSF_ERR_ARG, "output path too long");
                     std::strcpy(o->outFileName, v);
                     break;
+                case 'k':
+                    MMCSF_REQUIRE(std::strlen(v) < sizeof(o->cacheFileName), MMCSF_ERR_ARG, "cache path too long");
+                    std::strcpy(o->cacheFileName, v);
+                    break;
                 case 'p': std::snprintf(o->m0, sizeof(o->m0), "%s", v); break;
                 case 'q': std::snprintf(o->m1, sizeof(o->m1), "%s", v); break;
                 case 'r': std::snprintf(o->m2, sizeof(o->m2), "%s", v); break;
@@ -705,6 +750,19 @@ int mmcsf_format_build(const mmcsf_tensor *th, int type, const mmcsf_options *op
         const Tensor &t = th->t;
         // check_opt, util.h:245-248
         MMCSF_REQUIRE(opt->mode < (uint64_t)t.ndims, MMCSF_ERR_ARG, "Mode cannot be larger than tensor dimension.");
+        // The launch-shape flags (-w, -s, -b) do not shape this library's
+        // launches (the walk splits work by traversal cost); the limits the
+        // reference's launchers enforce on them are kept:
+        //   B-CSF / HB-CSF: -w * 32 <= 512 (mttkrp_gpu.cu:1961-1964, 2240-2243, 2660-2663)
+        //   MM-CSF / single CSF: -w, -s <= -b / 32 (mttkrp_gpu.cu:3624-3628, 3093-3097)
+        if (type == MMCSF_TYPE_BCSF || type == MMCSF_TYPE_BCSF_ANYMODE || type == MMCSF_TYPE_HBCSF)
+            MMCSF_REQUIRE(opt->warpPerSlice * 32 <= 512, MMCSF_ERR_ARG,
+                          "BLOCKSIZE is smaller than work per slice! Increase BLOCKSIZE.");
+        if (type == MMCSF_TYPE_MMCSF || type == MMCSF_TYPE_ONECSF)
+            MMCSF_REQUIRE(opt->warpPerSlice <= opt->TBsize / 32 && opt->fiberPerWarp <= opt->TBsize / 32,
+                          MMCSF_ERR_ARG,
+                          "warpPerFbr (-w) or fbrPerWarp (-s) cannot be higher than threadblock size!\n"
+                          "hint: increase -b!");
         hipStream_t s = static_cast<hipStream_t>(stream);
         auto fh = std::make_unique<mmcsf_format>();
         Format &f = fh->f;
@@ -797,6 +855,8 @@ int mmcsf_format_build(const mmcsf_tensor *th, int type, const mmcsf_options *op
                 }
             }
         }
+        // the first MTTKRP on this format runs without loading code objects
+        if (type != MMCSF_TYPE_COO_CPU) preload_kernels(type == MMCSF_TYPE_MMCSF || type == MMCSF_TYPE_BCSF);
         *out = fh.release();
     });
 }
@@ -971,6 +1031,47 @@ int mmcsf_format_export(const mmcsf_format *fh, const char *name, int k, void *d
 }
 
 void mmcsf_format_free(mmcsf_format *f) { delete f; }
+
+int mmcsf_format_save(const mmcsf_format *fh, const char *path) {
+    return guarded([&] {
+        MMCSF_REQUIRE(fh && path, MMCSF_ERR_ARG, "null argument");
+        format_save(fh->f, path);
+    });
+}
+
+int mmcsf_format_load(const char *path, mmcsf_format **out) {
+    return guarded([&] {
+        MMCSF_REQUIRE(path && out, MMCSF_ERR_ARG, "null argument");
+        auto fh = std::make_unique<mmcsf_format>();
+        format_load(path, fh->f);
+        *out = fh.release();
+    });
+}
+
+int mmcsf_format_is_cache(const char *path) { return path && is_format_cache(path) ? 1 : 0; }
+
+int mmcsf_format_to_tensor(const mmcsf_format *fh, mmcsf_tensor **out) {
+    return guarded([&] {
+        MMCSF_REQUIRE(fh && out, MMCSF_ERR_ARG, "null argument");
+        const Format &f = fh->f;
+        std::vector<std::vector<uint32_t>> inds;
+        std::vector<float> vals;
+        format_nonzeros(f, inds, vals);
+        MMCSF_REQUIRE(vals.size() == f.nnz, MMCSF_ERR_ARG, "format does not hold every nonzero");
+        auto th = std::make_unique<mmcsf_tensor>();
+        Tensor &t = th->t;
+        t.ndims = f.ndims;
+        t.nnz = f.nnz;
+        for (int m = 0; m < f.ndims; ++m) {
+            t.dims[m] = f.dims[m];
+            t.inds[m].alloc(f.nnz);
+            if (f.nnz) MMCSF_HIP(hipMemcpy(t.inds[m].p, inds[m].data(), f.nnz * 4, hipMemcpyHostToDevice));
+        }
+        t.vals.alloc(f.nnz);
+        if (f.nnz) MMCSF_HIP(hipMemcpy(t.vals.p, vals.data(), f.nnz * 4, hipMemcpyHostToDevice));
+        *out = th.release();
+    });
+}
 
 int mmcsf_mttkrp(const mmcsf_format *fh, int mode, int R, int dtype, const void *const *factors, float *out,
                  void *stream) {

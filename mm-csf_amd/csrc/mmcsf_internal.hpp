@@ -222,6 +222,10 @@ void mttkrp_mode(const Format &f, int mode, const Factors &F, float *out, hipStr
 bool walk_supports(const Csf &c, int R);
 void mttkrp_root(const Csf &c, const Factors &F, float *out, hipStream_t s, bool accumulate = false);
 void mttkrp_hyb(const Hyb &h, const uint64_t *dims, const Factors &F, float *out, hipStream_t s);
+// loads the code objects of the MTTKRP kernels (and, cp, of the CP
+// completion passes) on the current device, once per process
+void preload_kernels(bool cp);
+void preload_scatter_kernels();
 void fill_zero(float *p, uint64_t n, hipStream_t s);
 
 // TTM (walk.hip): Y[leaf fiber] = sum over its nonzeros of v * U_leaf[leaf]
@@ -235,6 +239,11 @@ void cp_gradient(const Csf &c, const Factors &F, const float *weights, float alp
                  hipStream_t s);
 void cp_half(double *loss, hipStream_t s);
 void cp_zero(double *loss, hipStream_t s);
+
+// binary cache of built formats (cache.cpp)
+void format_save(const Format &f, const char *path);
+void format_load(const char *path, Format &f);
+bool is_format_cache(const char *path);
 
 // CP-ALS (cpd.hip)
 void gram(const float *U, uint64_t rows, int R, double *G, hipStream_t s);

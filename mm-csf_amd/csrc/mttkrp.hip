@@ -465,6 +465,12 @@ void check_align(const void *p, int dtype, const Geo &g) {
 
 }  // namespace
 
+void preload_scatter_kernels() {
+    // one kernel of this translation unit: its code object loads with it
+    hipFuncAttributes attr;
+    MMCSF_HIP(hipFuncGetAttributes(&attr, reinterpret_cast<const void *>(k_zero)));
+}
+
 int device_cu_count() {
     static int cus = 0;
     if (!cus) {

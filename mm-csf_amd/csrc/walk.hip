@@ -1,6 +1,8 @@
 // Entry points of the chunked root walk (the kernel is in walk_kernel.hpp,
 // instantiated per operation in walk_mttkrp.hip, walk_loss.hip,
 // walk_lossgrad.hip, walk_wgrad.hip and walk_ttm.hip).
+#include <mutex>
+
 #include "walk.hpp"
 
 namespace mmcsf {
@@ -126,6 +128,27 @@ void ttm_fibers(const Csf &c, const Factors &F, float *Y, hipStream_t s) {
     WalkArgs a = args_of(c, F);
     a.out = Y;
     walk_launch<kTtm>(c, F, a, s);
+}
+
+void preload_kernels(bool cp) {
+    // once per device: the code objects of the traversals a format runs
+    static std::mutex lock;
+    static bool done[64][2];
+    int dev = 0;
+    MMCSF_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) return;
+    std::lock_guard<std::mutex> g(lock);
+    if (!done[dev][0]) {
+        walk_preload<kMttkrp>();
+        preload_scatter_kernels();
+        done[dev][0] = true;
+    }
+    if (cp && !done[dev][1]) {
+        walk_preload<kLossResid>();
+        walk_preload<kLossGrad>();
+        walk_preload<kWeightGrad>();
+        done[dev][1] = true;
+    }
 }
 
 void cp_half(double *loss, hipStream_t s) {

@@ -75,5 +75,7 @@ enum WalkOp : int {
 // own translation unit (walk_<op>.hip)
 template <int OP>
 void walk_launch(const Csf &c, const Factors &F, const WalkArgs &a, hipStream_t s);
+template <int OP>
+void walk_preload();
 
 }  // namespace mmcsf

@@ -868,6 +868,14 @@ void dispatch(const Cfg &c, const WalkArgs &a, hipStream_t s) {
 
 }  // namespace
 
+// loads this translation unit's code object on the current device (a kernel
+// attribute query), so that the first traversal does not pay for it
+template <int OP>
+void walk_preload() {
+    hipFuncAttributes attr;
+    MMCSF_HIP(hipFuncGetAttributes(&attr, reinterpret_cast<const void *>(k_walk<3, 8, 1, 8, float, 4, OP>)));
+}
+
 template <int OP>
 void walk_launch(const Csf &c, const Factors &F, const WalkArgs &a, hipStream_t s) {
     const Cfg cfg = pick(F.R, F.dtype);

@@ -3,4 +3,5 @@
 
 namespace mmcsf {
 template void walk_launch<kLossGrad>(const Csf &, const Factors &, const WalkArgs &, hipStream_t);
+template void walk_preload<kLossGrad>();
 }  // namespace mmcsf

@@ -45,6 +45,7 @@ class Options(C.Structure):
         ("cpdIters", C.c_uint64), ("natOrdering", C.c_int), ("fbrThreashold", C.c_uint64),
         ("factorDtype", C.c_int), ("seed", C.c_uint64), ("modeTrees", C.c_int),
         ("mixedPart", C.c_int), ("inFileName", C.c_char * 4096), ("outFileName", C.c_char * 4096),
+        ("cacheFileName", C.c_char * 4096),
         ("m0", C.c_char * 16), ("m1", C.c_char * 16), ("m2", C.c_char * 16),
     ]
 
@@ -103,6 +104,10 @@ def lib():
     L.mmcsf_format_build.argtypes = [VP, C.c_int, C.POINTER(Options), VP, C.POINTER(VP)]
     L.mmcsf_format_info_get.argtypes = [VP, C.POINTER(FormatInfo)]
     L.mmcsf_format_export.argtypes = [VP, C.c_char_p, C.c_int, VP, C.c_uint64, U64P]
+    L.mmcsf_format_save.argtypes = [VP, C.c_char_p]
+    L.mmcsf_format_load.argtypes = [C.c_char_p, C.POINTER(VP)]
+    L.mmcsf_format_is_cache.argtypes = [C.c_char_p]
+    L.mmcsf_format_to_tensor.argtypes = [VP, C.POINTER(VP)]
     L.mmcsf_format_free.argtypes = [VP]
     L.mmcsf_format_free.restype = None
     L.mmcsf_mttkrp.argtypes = [VP, C.c_int, C.c_int, C.c_int, C.POINTER(VP), VP, VP]
@@ -301,6 +306,26 @@ class Format:
         _check(lib().mmcsf_format_export(self._h, name.encode(), int(k), out.ctypes.data_as(VP),
                                          n.value, C.byref(n)))
         return out[: n.value]
+
+    def save(self, path):
+        """Binary cache of the built format (mmcsf_format_save)."""
+        _check(lib().mmcsf_format_save(self._h, str(path).encode()))
+
+    @classmethod
+    def load(cls, path):
+        """A format from its binary cache (mmcsf_format_load), without building."""
+        obj = cls.__new__(cls)
+        h = VP()
+        _check(lib().mmcsf_format_load(str(path).encode(), C.byref(h)))
+        obj._h = h
+        obj.type = obj.info()["type"]
+        return obj
+
+    def to_tensor(self):
+        """The nonzeros the format stores, as a Tensor (mmcsf_format_to_tensor)."""
+        h = VP()
+        _check(lib().mmcsf_format_to_tensor(self._h, C.byref(h)))
+        return Tensor(h)
 
     def free(self):
         if self._h is not None and self._h.value:

@@ -1,10 +1,13 @@
 // mttkrp -- command-line driver with the reference's flags and output lines
 // (src/mttkrp.cu:71-472), running every GPU type on gfx950 through the C ABI.
 //
-//   mttkrp -i X.tns [-m mode] [-R rank] [-t 1|3|8|10|12] [-f fbrThreshold]
-//          [-w w] [-b b] [-s s] [-c 1] [-o out.txt] [-v 1] [-d 0|1]
+//   mttkrp -i X.tns|X.fmt [-m mode] [-R rank] [-t 1|3|8|9|10|12|14] [-f fbrThreshold]
+//          [-w w] [-b b] [-s s] [-c 1] [-o out.txt] [-v 1] [-d 0|1] [-M 0|1] [-k X.fmt]
 //
-// -d 1 (new) stores the factors in bfloat16.  Factors are the reference's
+// -d 1 (new) stores the factors in bfloat16.  -M 1 (new) builds MM-CSF with
+// the reference's mixed-mode partition heuristic instead of its override.
+// -k (new) writes the built format to a binary cache; -i then accepts that
+// cache in place of the .tns text (no text parsing, sorting or building).  Factors are the reference's
 // constants U[m] = m + 0.5 (util.h:3220-3223).  Errors print a message and
 // exit with a non-zero status (the reference exits 0 on most of them).
 #include <hip/hip_runtime.h>
@@ -59,7 +62,11 @@ void print_help() {
         "        -m mode : set the mode of MTTKRP (default 0, MMCSF evaluates all modes)\n"
         "        -t implementation type: 1: COO CPU, 3: COO GPU 8: B-CSF 10: HB-CSF 12: MM-CSF on GPU (default 1)\n"
         "        -f fiber-splitting threshold: maximum nonzeros per fiber (default inf)\n"
-        "        -w warp per slice (default 4)\n"
+        "        -w warp per slice (default 4); -w, -s, -b and -g are checked against the reference's limits\n"
+        "           and do not shape the launch (the chunked walk splits work by traversal cost)\n"
+        "        -h MM-CSF fiber threshold of the mixed-mode partition (default 1)\n"
+        "        -M 1: MM-CSF with the mixed-mode partition heuristic (default 0: every nonzero in partition 2)\n"
+        "        -k format cache file to write after building; -i accepts such a file\n"
         "        -d factor storage: 0 float32, 1 bfloat16 (default 0)\n"
         "        -c correctness check against COO CPU on the last mode: 1 to enable\n"
         "        -i input file name: e.g., ../dataset/delicious.tns\n"
@@ -120,7 +127,14 @@ int main(int argc, char **argv) {
         return 1;
     }
     mmcsf_tensor *X = nullptr;
-    CK(mmcsf_tensor_load_tns(opt.inFileName, &X));
+    mmcsf_format *cachedF = nullptr;
+    if (mmcsf_format_is_cache(opt.inFileName)) {
+        // a format cache: the format as built, and the nonzeros it holds
+        CK(mmcsf_format_load(opt.inFileName, &cachedF));
+        CK(mmcsf_format_to_tensor(cachedF, &X));
+    } else {
+        CK(mmcsf_tensor_load_tns(opt.inFileName, &X));
+    }
     int ndims = 0;
     uint64_t dims[4] = {0, 0, 0, 0}, nnz = 0;
     CK(mmcsf_tensor_info(X, &ndims, dims, &nnz));
@@ -130,6 +144,24 @@ int main(int argc, char **argv) {
     }
     const int type = (int)opt.impType;
     if (type == 11 || type == 12) opt.mode = 0;  // check_opt, util.h:250-251
+    // the format of -t: the cached one, or built from X (and cached with -k)
+    auto make_format = [&](const mmcsf_options &o) -> mmcsf_format * {
+        mmcsf_format *F = nullptr;
+        if (cachedF) {
+            mmcsf_format_info ci;
+            CK(mmcsf_format_info_get(cachedF, &ci));
+            if (ci.type != type) {
+                std::fprintf(stderr, "the format cache holds a -t %d format, not -t %d\n", ci.type, type);
+                std::exit(1);
+            }
+            F = cachedF;
+            cachedF = nullptr;
+            return F;
+        }
+        CK(mmcsf_format_build(X, type, &o, nullptr, &F));
+        if (opt.cacheFileName[0]) CK(mmcsf_format_save(F, opt.cacheFileName));
+        return F;
+    };
     const uint64_t R = opt.R;
     const int dtype = opt.factorDtype;
 
@@ -157,8 +189,7 @@ int main(int argc, char **argv) {
     int resultMode = (int)opt.mode;
 
     if (type == MMCSF_TYPE_COO_CPU) {
-        mmcsf_format *F = nullptr;
-        CK(mmcsf_format_build(X, MMCSF_TYPE_COO_CPU, &opt, nullptr, &F));
+        mmcsf_format *F = make_format(opt);
         std::vector<const float *> Up(ndims);
         for (int m = 0; m < ndims; ++m) Up[m] = Uh[m].data();
         std::vector<float> out = Uh[opt.mode];  // not zeroed (mttkrp.cu:88-89)
@@ -168,9 +199,8 @@ int main(int argc, char **argv) {
         result = out;
         mmcsf_format_free(F);
     } else if (type == MMCSF_TYPE_COO || type == MMCSF_TYPE_BCSF || type == MMCSF_TYPE_HBCSF) {
-        mmcsf_format *F = nullptr;
         double tb = seconds();
-        CK(mmcsf_format_build(X, type, &opt, nullptr, &F));
+        mmcsf_format *F = make_format(opt);
         if (opt.verbose) std::printf("GPU format build - time: %.3f sec \n", seconds() - tb);
         mmcsf_format_info info;
         CK(mmcsf_format_info_get(F, &info));
@@ -193,9 +223,8 @@ int main(int argc, char **argv) {
         resultMode = mode;
         mmcsf_format_free(F);
     } else if (type == MMCSF_TYPE_MMCSF) {
-        mmcsf_format *F = nullptr;
         double t0 = seconds();
-        CK(mmcsf_format_build(X, MMCSF_TYPE_MMCSF, &opt, nullptr, &F));
+        mmcsf_format *F = make_format(opt);
         std::printf("mm_partition & populate - time: %.3f sec \n", seconds() - t0);
         mmcsf_format_info info;
         CK(mmcsf_format_info_get(F, &info));
@@ -221,9 +250,8 @@ int main(int argc, char **argv) {
         mmcsf_format_free(F);
     } else if (type == MMCSF_TYPE_BCSF_ANYMODE || type == MMCSF_TYPE_ONECSF) {
         // one tree, every mode (mttkrp.cu:223-231, 234-299)
-        mmcsf_format *F = nullptr;
         double tb = seconds();
-        CK(mmcsf_format_build(X, type, &opt, nullptr, &F));
+        mmcsf_format *F = make_format(opt);
         if (opt.verbose) std::printf("GPU format build - time: %.3f sec \n", seconds() - tb);
         std::vector<float> ms(ndims, 0.f);
         CK(mmcsf_mttkrp_allmode(F, (int)R, dtype, Ud.data(), Od.data(), nullptr, ms.data()));
@@ -266,6 +294,7 @@ int main(int argc, char **argv) {
         (void)hipFree(Ud[m]);
         (void)hipFree(Od[m]);
     }
+    if (cachedF) mmcsf_format_free(cachedF);
     mmcsf_tensor_free(X);
     return 0;
 }
