@@ -32,13 +32,7 @@ for _p in (ROOT, os.path.join(ROOT, "mm-csf_amd")):
 METRIC = "All-mode MTTKRP ms & HBM GB/s, FP32 vs BF16-factor at <=1e-3 rel err"
 HBM_PEAK_GBS = 8000.0
 SEED = 0x4D4D43534600 + 1
-# Random factor-row gather ceiling of one MI355X: 128-byte rows (float32,
-# R=32) gathered by groups of 8 lanes x 16 B, uniformly at random from a
-# 6.4 MB table (the leaf table of the mode-0 and mode-1 trees of this tensor),
-# by a kernel that does nothing else: 162 G rows/s (profiles/gather_ceiling.md).
-GATHER_CEILING_G_ROWS_F32 = 162.0
-GATHER_NOTE = ("random 128-byte row gathers per second of a gather-only kernel on a 6.4 MB table "
-               "(profiles/gather_ceiling.md)")
+GATHER_TOOL = os.path.join(ROOT, "mm-csf_amd", "bin", "gather_ceiling")
 
 
 def log(msg):
@@ -152,6 +146,36 @@ def cpu_baseline(torch, inds, vals, U, dims, bytes_f32):
         "cpu_model": cpu_model(),
         "host_cores_available": os.cpu_count(),
     }
+
+
+def gather_ceiling(table_mb):
+    """Random row-gather rates of this GPU from the gather-only kernels of
+    mm-csf_amd/tools/gather_ceiling.hip: 128- and 64-byte rows (float32 and
+    bfloat16 rows at R=32) uniformly at random from 1 MB and 3.2 MB tables
+    (resident in every XCD's 4 MB L2; the better of the two is the rate no
+    gather of such rows exceeds) and from a
+    table of table_mb (uniform rows over the whole factor table: the rate
+    without the locality a skewed tensor gives), and the coalesced
+    stream-read rate."""
+    import subprocess
+    if not os.path.exists(GATHER_TOOL):
+        raise RuntimeError(f"{GATHER_TOOL} not built (run make -C mm-csf_amd)")
+    r = subprocess.run([GATHER_TOOL, "-b", "128,64", "-t", f"1,3.2,{table_mb:.3f}"], capture_output=True,
+                       text=True, timeout=120, check=True)
+    lines = [json.loads(x) for x in r.stdout.splitlines() if x.startswith("{")]
+    out = {"leaf_table_mb": round(table_mb, 3)}
+    leaf = round(table_mb, 3)
+    for d in lines:
+        if d["kind"] == "stream":
+            out["stream_gb_per_s"] = d["gb_per_s"]
+        if d["kind"] != "gather":
+            continue
+        rb, t, v = d["row_bytes"], d["table_mb"], d["g_rows_per_s"]
+        if t in (1.0, 3.2):  # L2-resident: the better of the two
+            out[f"g_rows_per_s_{rb}B_l2"] = max(out.get(f"g_rows_per_s_{rb}B_l2", 0.0), v)
+        if abs(t - leaf) < 1e-6:
+            out[f"g_rows_per_s_{rb}B_leaf_table"] = v
+    return out
 
 
 def gathered_rows(info, single_tree=False):
@@ -278,6 +302,10 @@ def run_c2(args):
 
     dom = max(range(3), key=lambda m: f32_k[m])
     achieved = bytes_f32[dom] / (f32_k[dom] * 1e-3) / 1e9
+    leaf = info["modeTreeOrder"][dom][-1]
+    log("gather ceiling")
+    ceil = gather_ceiling(dims[leaf] * R * 4 / 1e6)
+    log(f"gather ceiling: {ceil}")
     flops = 3 * args.nnz * R * 3
     gbytes = lambda b: round(b / 1e9, 3)  # noqa: E731
 
@@ -303,7 +331,7 @@ def run_c2(args):
                          "-w 1 and -w 4 give the same launch (the walk splits work by chunk cost, not per slice)"),
             "R": R, "nnz": args.nnz, "dims": dims, "fbr_threshold": 128, "layout": "-a 1",
             "structure_bytes": {"-a 1": info["deviceBytes"], "-a 0": info1["deviceBytes"]},
-            "mode_order": info["modeOrder"], "n_slices": info["nLevel"][0],
+            "mode_order": info["modeOrder"], "tree_orders": info["modeTreeOrder"], "n_slices": info["nLevel"][0],
             "n_fibers": info["nLevel"][1], "mode_tree_levels": info["modeLevel"], "distinct_rows": distinct,
         },
         "gflops": round(flops / (f32_ms * 1e-3) / 1e9, 1),
@@ -315,6 +343,7 @@ def run_c2(args):
             "ms_per_step": round(b16_ms, 4),
             "gbs": round(sum(bytes_bf16) / (b16_ms * 1e-3) / 1e9, 2),
             "speedup_vs_f32": round(f32_ms / b16_ms, 3),
+            "gather_ceiling_64B_over_128B": round(ceil["g_rows_per_s_64B_l2"] / ceil["g_rows_per_s_128B_l2"], 3),
             "modes": [{"mode": m, "kernel_ms": round(b16_k[m], 4), "alg_bytes": bytes_bf16[m],
                        "g_rows_per_s": round(rows[m] / (b16_k[m] * 1e-3) / 1e9, 1),
                        "rel_err_fp64": err16[m][0], "max_rel_err_fp64": err16[m][1]} for m in range(3)],
@@ -334,9 +363,12 @@ def run_c2(args):
             "gather": {
                 "rows_per_launch": rows[dom],
                 "achieved_g_rows_per_s": round(rows[dom] / (f32_k[dom] * 1e-3) / 1e9, 1),
-                "ceiling_g_rows_per_s": GATHER_CEILING_G_ROWS_F32,
-                "frac": round(rows[dom] / (f32_k[dom] * 1e-3) / 1e9 / GATHER_CEILING_G_ROWS_F32, 4),
-                "note": GATHER_NOTE,
+                "ceiling_g_rows_per_s": ceil["g_rows_per_s_128B_l2"],
+                "frac": round(rows[dom] / (f32_k[dom] * 1e-3) / 1e9 / ceil["g_rows_per_s_128B_l2"], 4),
+                "ceiling": ceil,
+                "note": ("ceiling: 128-byte rows gathered at random from an L2-resident table by a kernel that "
+                         "does nothing else (mm-csf_amd/bin/gather_ceiling, measured in this run); uniform rows "
+                         "over a table of the leaf factor table's size gather at the *_leaf_table rate"),
             },
         },
         "single_tree": {
@@ -429,9 +461,12 @@ def run_next(args):
     """The rows beyond the five MTTKRP types, each timed with its parity check:
     (1) the parallel .tns reader and the binary COO cache against the
     single-threaded reader of the oracle (load_tensor, util.h:313-370);
-    (2) the one-tree all-mode types -t 9 / -t 14 against float64, beside the
-    default -t 12 (a root tree per mode); (3) leaf-mode TTM (ttm.cu) against
-    float64 and the sequential host TTM (TTM_CPU, ttm_cpu.cpp:60-86)."""
+    (2) the one-tree all-mode types -t 9 / -t 13 / -t 14 and the MM-CSF variants
+    (-a 0: the reference's one tree; -M 1: the mixed-mode partition) against
+    float64, beside the default -t 12 (a root tree per mode), with build times,
+    and the binary format cache (save, load, bit-identical outputs); (3)
+    leaf-mode TTM (ttm.cu) against float64 and the sequential host TTM
+    (TTM_CPU, ttm_cpu.cpp:60-86)."""
     import tempfile
 
     import numpy as np
@@ -478,13 +513,50 @@ def run_next(args):
     refs = [fp64_reference(inds, vals, U, m, dims) for m in range(3)]
     outs = [torch.empty((d, R), dtype=torch.float32, device=dev) for d in dims]
     types = {}
-    for typ, name in ((12, "MM-CSF, a root tree per mode"), (9, "ONE-B-CSF"), (14, "single CSF")):
-        fmt = mmcsf.Format(X, typ, fbr_threshold=128)
+    cache = None
+    for key, typ, name, kw in (("12", 12, "MM-CSF, a root tree per mode (-a 1)", {}),
+                               ("12 -a 0", 12, "MM-CSF, the reference's one tree (-a 0)", {"mode_trees": False}),
+                               ("12 -M 1", 12, "MM-CSF, mixed-mode partition (-M 1)", {"mixed": True}),
+                               ("9", 9, "ONE-B-CSF", {}), ("13", 13, "single CSF, no bins", {}),
+                               ("14", 14, "single CSF", {})):
+        t0 = time.perf_counter()
+        fmt = mmcsf.Format(X, typ, fbr_threshold=128, **kw)
+        torch.cuda.synchronize()
+        build_s = time.perf_counter() - t0
         ms = _event_ms(torch, lambda: mmcsf.mttkrp_allmode(fmt, U, outs), args.steps, args.warmup)
         errs = [rel_err(outs[m], refs[m])[0] for m in range(3)]
-        types[str(typ)] = {"name": name, "all_mode_ms": round(ms, 4), "rel_err_fp64": errs}
-        log(f"next -t {typ}: {ms:.3f} ms, errs {errs}")
+        info = fmt.info()
+        types[key] = {"name": name, "type": typ, "all_mode_ms": round(ms, 4), "build_s": round(build_s, 3),
+                      "structure_bytes": info["deviceBytes"], "rel_err_fp64": errs}
+        if kw.get("mixed"):
+            types[key]["partition_nnz"] = info["partNnz"][:3]
+        log(f"next -t {key}: {types[key]}")
+        if key == "12":
+            # binary cache of the built format: save, reload, same outputs bit for bit
+            want = [o.clone() for o in outs]
+            with tempfile.TemporaryDirectory() as d:
+                p_fmt = os.path.join(d, "x.fmt")
+                t0 = time.perf_counter()
+                fmt.save(p_fmt)
+                save_s = time.perf_counter() - t0
+                size = os.path.getsize(p_fmt)
+                t0 = time.perf_counter()
+                f2 = mmcsf.Format.load(p_fmt)
+                torch.cuda.synchronize()
+                load_s = time.perf_counter() - t0
+            mmcsf.mttkrp_allmode(f2, U, outs)
+            torch.cuda.synchronize()
+            # slices spanning waves are added with float atomics, so two runs
+            # of one format agree to rounding, not bit for bit (the cache's
+            # arrays round-trip bit for bit; the GPU tests check that)
+            cache = {"file_MB": round(size / 1e6, 1), "save_s": round(save_s, 3), "load_s": round(load_s, 3),
+                     "build_s": round(build_s, 3), "load_vs_build": round(build_s / load_s, 2),
+                     "rel_diff_vs_built": max(rel_err(outs[m], want[m].double())[0] for m in range(3)),
+                     "rel_err_fp64": [rel_err(outs[m], refs[m])[0] for m in range(3)]}
+            log(f"next format cache: {cache}")
+            del f2, want
         del fmt
+        torch.cuda.empty_cache()
     del refs
 
     # (3) TTM on the leaf mode of the B-CSF tree rooted at mode 0
@@ -513,11 +585,11 @@ def run_next(args):
            "host_cores": 1, "host_rel_err_fp64": errh,
            "gbs": round((nnz * 8 + nf * 8 + nf * R * 4) / (ms * 1e-3) / 1e9, 1)}
     log(f"next ttm: {ttm}")
-    _line("Rows beyond the MTTKRP types: .tns reader, one-tree types, TTM", types["9"]["all_mode_ms"], "ms",
-          types["9"]["all_mode_ms"], args, "f32",
-          {"workload": f"reader: 2000^3 uniform 2M nonzeros; one-tree types and TTM: 500000x200000x50000, "
-                       f"{args.nnz_next} nonzeros, Zipf1.2 x Zipf1.2 x uniform, R=32, -f 128",
-           "R": R, "nnz": args.nnz_next}, higher=False, io=io, types=types, ttm=ttm)
+    _line("Rows beyond the MTTKRP types: .tns reader, one-tree and MM-CSF variants, format cache, TTM",
+          types["9"]["all_mode_ms"], "ms", types["9"]["all_mode_ms"], args, "f32",
+          {"workload": f"reader: 2000^3 uniform 2M nonzeros; all-mode variants, format cache and TTM: "
+                       f"500000x200000x50000, {args.nnz_next} nonzeros, Zipf1.2 x Zipf1.2 x uniform, R=32, -f 128",
+           "R": R, "nnz": args.nnz_next}, higher=False, io=io, types=types, format_cache=cache, ttm=ttm)
 
 
 def run_c3(args):

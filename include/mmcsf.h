@@ -40,6 +40,10 @@ extern "C" {
                                      (MTTKRP_B_HCSR_GPU_ANYMODE, mttkrp.cu:223-231) */
 #define MMCSF_TYPE_HBCSF 10       /* -t 10 HB-CSF (COO + CSL + CSF parts) */
 #define MMCSF_TYPE_MMCSF 12       /* -t 12 MM-CSF, every mode from one build */
+#define MMCSF_TYPE_SINGLECSF 13   /* -t 13 single untiled CSF, every mode from it
+                                     (mttkrp.cu:234-262; the reference computes it on
+                                     the host with MTTKRP_HCSR_CPU[_4D] / _mode1 /
+                                     _mode2, mttkrp_cpu.cpp:160-348, here on the GPU) */
 #define MMCSF_TYPE_ONECSF 14      /* -t 14 single CSF on the GPU, every mode from it
                                      (MTTKRP_ONE_HCSR_GPU, mttkrp.cu:234-299) */
 
@@ -163,6 +167,9 @@ void mmcsf_tensor_free(mmcsf_tensor *t);
  *                    make_TiledBin + create_fbrLikeSlcInds (mttkrp.cu:153-219);
  *                    MMCSF_TYPE_BCSF_ANYMODE and MMCSF_TYPE_ONECSF build the
  *                    same arrays (mttkrp.cu:153-231, 234-299)
+ *   MMCSF_TYPE_SINGLECSF sort_COOtensor + create_HCSR + create_fbrLikeSlcInds
+ *                    (mttkrp.cu:234-262): the arrays of MMCSF_TYPE_BCSF
+ *                    without the slice bins
  *   MMCSF_TYPE_HBCSF create_HCSR + create_HYB[_4D] + make_HybBin
  *                    (mttkrp.cu:138-150, util.h:852-1142, 1786-1885)
  *   MMCSF_TYPE_MMCSF mm_partition_reuseBased + populate_paritions +
@@ -197,6 +204,10 @@ typedef struct mmcsf_format_info {
                                   costs, bins and hot-row tables) */
     uint64_t partNnz[4];       /* MM-CSF: nonzeros per partition (X.totnnzPerPart) */
     int mixedPart;             /* MM-CSF built with -M 1 */
+    int modeTreeOrder[4][4];   /* MM-CSF and one-tree types: the mode order of the tree
+                                  the MTTKRP of mode m traverses (level l holds mode
+                                  modeTreeOrder[m][l]; its root is m when that tree is
+                                  a root tree); -M 1: of partition m, as modeLevel */
 } mmcsf_format_info;
 
 int mmcsf_format_info_get(const mmcsf_format *f, mmcsf_format_info *info);
@@ -264,18 +275,21 @@ int mmcsf_format_to_tensor(const mmcsf_format *f, mmcsf_tensor **out);
  *   MM-CSF MTTKRP_MIHCSR_GPU[_oneMode_forCPD]
  *                                    (mttkrp_gpu.h:75-81, mttkrp_gpu.cu:3380, 3860)
  * COO, B-CSF and HB-CSF serve their root mode (modeOrder[0]) only, like
- * the reference; MM-CSF and the one-tree types 9 / 14 serve every mode
- * (types 9 / 14: the root mode walks the tree, the other modes scatter from
- * their level, mttkrp_gpu.cu:2131-2528, 2853-3378).
+ * the reference; MM-CSF and the one-tree types 9 / 13 / 14 serve every mode
+ * (types 9 / 13 / 14: the root mode walks the tree, the other modes scatter
+ * from their level, mttkrp_gpu.cu:2131-2528, 2853-3378, and for type 13 the
+ * host routines MTTKRP_HCSR_CPU[_4D] / _mode1 / _mode2, mttkrp_cpu.cpp:160-348,
+ * whose 3-D-only mode1 / mode2 roles this generalises to 4-D: every mode of a
+ * 4-D type-13 format is its exact MTTKRP).
  */
 int mmcsf_mttkrp(const mmcsf_format *f, int mode, int R, int dtype,
                  const void *const *factors, float *out, void *stream);
 
 /*
- * All-mode MTTKRP of an MM-CSF or one-tree (types 9 / 14) format
+ * All-mode MTTKRP of an MM-CSF or one-tree (types 9 / 13 / 14) format
  * (MTTKRP_MIHCSR_GPU's mode loop, mttkrp_gpu.cu:3548-3744; the mode loops of
  * mttkrp.cu:225-230, 244-298): outs[m] receives the MTTKRP of mode m for every
- * m in 0..ndims-1, all from the same factors.  Optional ms_per_mode (host,
+ * m in 0..ndims-1, all from the same factors (type 13: mttkrp.cu:244-262).  Optional ms_per_mode (host,
  * ndims entries) receives each mode's MTTKRP kernel time (after the output
  * has been zeroed), measured with HIP events on `stream`; the call then
  * synchronises on the last event (NULL: no timing, no synchronisation).
@@ -297,7 +311,7 @@ int mmcsf_mttkrp_coo_host(const mmcsf_format *f, int mode, int R,
 
 /*
  * Tensor-times-matrix on the leaf mode of a B-CSF-family format (types 8, 9,
- * 14: the tree of create_HCSR with the -f split, ttm.cu:76-78) -- TTM_CPU
+ * 13, 14: the tree of create_HCSR with the -f split, ttm.cu:76-78) -- TTM_CPU
  * (ttm_cpu.cpp:60-86) and TTM_GPU (ttm_gpu.cu:140-259):
  *     Y[f][r] = sum over the nonzeros x of leaf fiber f of
  *               v_x * U[modeOrder[ndims-1]][leaf_x][r].

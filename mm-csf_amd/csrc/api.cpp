@@ -272,10 +272,18 @@ void format_nonzeros(const Format &f, std::vector<std::vector<uint32_t>> &inds, 
     }
 }
 
+// one tree built by create_HCSR with the -f split, every mode served from it
+bool one_tree_type(int type) {
+    return type == MMCSF_TYPE_BCSF_ANYMODE || type == MMCSF_TYPE_SINGLECSF || type == MMCSF_TYPE_ONECSF;
+}
+
+// the tree of create_HCSR (-t 8 and the one-tree types)
+bool bcsf_family(int type) { return type == MMCSF_TYPE_BCSF || one_tree_type(type); }
+
 bool mmcsf_type_ok(int type) {
     return type == MMCSF_TYPE_COO_CPU || type == MMCSF_TYPE_COO || type == MMCSF_TYPE_BCSF ||
-           type == MMCSF_TYPE_BCSF_ANYMODE || type == MMCSF_TYPE_ONECSF || type == MMCSF_TYPE_HBCSF ||
-           type == MMCSF_TYPE_MMCSF;
+           type == MMCSF_TYPE_BCSF_ANYMODE || type == MMCSF_TYPE_ONECSF || type == MMCSF_TYPE_SINGLECSF ||
+           type == MMCSF_TYPE_HBCSF || type == MMCSF_TYPE_MMCSF;
 }
 
 int redundant_mode(int ndims, const uint64_t *dims) {
@@ -746,7 +754,7 @@ int mmcsf_format_build(const mmcsf_tensor *th, int type, const mmcsf_options *op
     return guarded([&] {
         MMCSF_REQUIRE(th && opt && out, MMCSF_ERR_ARG, "null argument");
         MMCSF_REQUIRE(mmcsf_type_ok(type), MMCSF_ERR_UNSUPPORTED,
-                      "format type must be one of 1, 3, 8, 10, 12");
+                      "format type must be one of 1, 3, 8, 9, 10, 12, 13, 14");
         const Tensor &t = th->t;
         // check_opt, util.h:245-248
         MMCSF_REQUIRE(opt->mode < (uint64_t)t.ndims, MMCSF_ERR_ARG, "Mode cannot be larger than tensor dimension.");
@@ -839,14 +847,15 @@ int mmcsf_format_build(const mmcsf_tensor *th, int type, const mmcsf_options *op
                 for (int l = 0; l < t.ndims; ++l) f.cooOrder[l] = order[l];
                 sort_coo(t, order, f.cooKeys, f.cooVals, s);
             } else {
-                const bool bcsf = type != MMCSF_TYPE_HBCSF;  // types 8, 9, 14
+                const bool bcsf = type != MMCSF_TYPE_HBCSF;  // types 8, 9, 13, 14
                 DevArray<uint32_t> pos;
                 sort_coo(t, order, keys, vals, s, bcsf ? &pos : nullptr);
                 build_csf(t.ndims, t.dims, t.nnz, order, keys, std::move(vals), opt->fbrThreashold, kNoSplit,
                           f.csf, s);
                 if (bcsf) {
                     f.csf.pos = std::move(pos);
-                    build_bins_tiled(f.csf, s);
+                    // type 13 is create_HCSR alone (mttkrp.cu:244-247): no slice bins
+                    if (type != MMCSF_TYPE_SINGLECSF) build_bins_tiled(f.csf, s);
                     // one-tree all-mode types: the other modes scatter from their level
                     if (type != MMCSF_TYPE_BCSF)
                         for (int l = 1; l < t.ndims; ++l) build_hot_table(f.csf, l, t.dims[order[l]], f.csf.hot[l], s);
@@ -899,17 +908,23 @@ int mmcsf_format_info_get(const mmcsf_format *fh, mmcsf_format_info *info) {
             info->droppedSlices = c->dropped;
         }
         for (int m = 0; m < f.ndims; ++m) info->modeTrees += f.hasTree[m] ? 1 : 0;
-        if (f.type == MMCSF_TYPE_MMCSF || f.type == MMCSF_TYPE_BCSF_ANYMODE || f.type == MMCSF_TYPE_ONECSF)
+        if (f.type == MMCSF_TYPE_MMCSF || one_tree_type(f.type))
             for (int m = 0; m < f.ndims; ++m) {
                 const Csf &t = f.hasTree[m] ? f.tree[m] : f.csf;
-                for (int l = 0; l < f.ndims; ++l) info->modeLevel[m][l] = t.cnt[l];
+                for (int l = 0; l < f.ndims; ++l) {
+                    info->modeLevel[m][l] = t.cnt[l];
+                    info->modeTreeOrder[m][l] = t.order[l];
+                }
             }
         info->deviceBytes = format_bytes(f);
         for (int p = 0; p < f.ndims; ++p) info->partNnz[p] = f.mixed ? f.partNnz[p] : (p == 2 ? f.nnz : 0);
         info->mixedPart = f.mixed ? 1 : 0;
-        if (f.mixed)
+        if (f.mixed)  // per partition m (an empty partition reports zeros)
             for (int m = 0; m < f.ndims; ++m)
-                for (int l = 0; l < f.ndims; ++l) info->modeLevel[m][l] = f.part[m].cnt[l];
+                for (int l = 0; l < f.ndims; ++l) {
+                    info->modeLevel[m][l] = f.part[m].cnt[l];
+                    info->modeTreeOrder[m][l] = f.partNnz[m] ? f.part[m].order[l] : 0;
+                }
     });
 }
 
@@ -1111,8 +1126,8 @@ int mmcsf_mttkrp_allmode(const mmcsf_format *fh, int R, int dtype, const void *c
     return guarded([&] {
         MMCSF_REQUIRE(fh && factors && outs, MMCSF_ERR_ARG, "null argument");
         const Format &f = fh->f;
-        MMCSF_REQUIRE(f.type == MMCSF_TYPE_MMCSF || f.type == MMCSF_TYPE_BCSF_ANYMODE || f.type == MMCSF_TYPE_ONECSF,
-                      MMCSF_ERR_ARG, "all-mode MTTKRP needs an MM-CSF or one-tree (-t 9 / 14) format");
+        MMCSF_REQUIRE(f.type == MMCSF_TYPE_MMCSF || one_tree_type(f.type), MMCSF_ERR_ARG,
+                      "all-mode MTTKRP needs an MM-CSF or one-tree (-t 9 / 13 / 14) format");
         MMCSF_REQUIRE(dtype == MMCSF_F32 || dtype == MMCSF_BF16, MMCSF_ERR_ARG, "dtype must be F32 or BF16");
         hipStream_t s = static_cast<hipStream_t>(stream);
         // one event pair per mode, around the MTTKRP kernel only
@@ -1223,8 +1238,7 @@ int mmcsf_gram(const float *U, uint64_t rows, int R, double *G, void *stream) {
 
 namespace {
 const Csf *ttm_tree(const Format &f) {
-    MMCSF_REQUIRE(f.type == MMCSF_TYPE_BCSF || f.type == MMCSF_TYPE_BCSF_ANYMODE || f.type == MMCSF_TYPE_ONECSF,
-                  MMCSF_ERR_ARG, "TTM needs a B-CSF format (-t 8, 9 or 14)");
+    MMCSF_REQUIRE(bcsf_family(f.type), MMCSF_ERR_ARG, "TTM needs a B-CSF format (-t 8, 9, 13 or 14)");
     return &f.csf;
 }
 }  // namespace
@@ -1285,9 +1299,7 @@ const Csf *root_tree(const Format &f, int mode) {
         if (f.hasTree[mode]) return &f.tree[mode];
         return nullptr;
     }
-    if ((f.type == MMCSF_TYPE_BCSF || f.type == MMCSF_TYPE_BCSF_ANYMODE || f.type == MMCSF_TYPE_ONECSF) &&
-        f.csf.order[0] == mode)
-        return &f.csf;
+    if (bcsf_family(f.type) && f.csf.order[0] == mode) return &f.csf;
     return nullptr;
 }
 
@@ -1317,9 +1329,8 @@ int mmcsf_cp_forward(const mmcsf_format *fh, int R, int dtype, const void *const
     return guarded([&] {
         MMCSF_REQUIRE(fh && loss, MMCSF_ERR_ARG, "null argument");
         const Format &f = fh->f;
-        MMCSF_REQUIRE(f.type == MMCSF_TYPE_MMCSF || f.type == MMCSF_TYPE_BCSF || f.type == MMCSF_TYPE_BCSF_ANYMODE ||
-                          f.type == MMCSF_TYPE_ONECSF,
-                      MMCSF_ERR_ARG, "CP completion needs a B-CSF or MM-CSF format");
+        MMCSF_REQUIRE(f.type == MMCSF_TYPE_MMCSF || bcsf_family(f.type), MMCSF_ERR_ARG,
+                      "CP completion needs a B-CSF or MM-CSF format");
         const Factors F = cp_factors(f, R, dtype, factors);
         MMCSF_REQUIRE(f.csf.pos.p || f.nnz == 0, MMCSF_ERR_ARG, "format holds no nonzero positions");
         cp_forward(f.csf, F, residual, loss, static_cast<hipStream_t>(stream));
@@ -1356,9 +1367,8 @@ int mmcsf_cp_loss_grad(const mmcsf_format *fh, int R, int dtype, const void *con
             MMCSF_REQUIRE(reinterpret_cast<uintptr_t>(grads[m]) % 16 == 0, MMCSF_ERR_ARG,
                           "gradient pointer is not aligned to 16 bytes");
         }
-        MMCSF_REQUIRE(f.type == MMCSF_TYPE_MMCSF || f.type == MMCSF_TYPE_BCSF || f.type == MMCSF_TYPE_BCSF_ANYMODE ||
-                          f.type == MMCSF_TYPE_ONECSF,
-                      MMCSF_ERR_ARG, "CP completion needs a B-CSF or MM-CSF format");
+        MMCSF_REQUIRE(f.type == MMCSF_TYPE_MMCSF || bcsf_family(f.type), MMCSF_ERR_ARG,
+                      "CP completion needs a B-CSF or MM-CSF format");
         MMCSF_REQUIRE(f.csf.pos.p || f.nnz == 0, MMCSF_ERR_ARG, "format holds no nonzero positions");
         hipStream_t s = static_cast<hipStream_t>(stream);
         cp_zero(loss, s);

@@ -9,6 +9,7 @@
 // element count and its elements), then the 8-byte end mark "MMCSFEND".
 // Loading checks the magic, version, byte order, every count against the
 // remaining file size and the end mark.
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -25,8 +26,47 @@ constexpr char kEnd[8] = {'M', 'M', 'C', 'S', 'F', 'E', 'N', 'D'};
 constexpr uint32_t kVersion = 1;
 constexpr uint32_t kByteOrder = 0x01020304u;
 
+// Two pinned host buffers between the file and the device: the file read or
+// write of one chunk overlaps the copy of the next on a stream of its own.
+struct Stage {
+    static constexpr size_t kChunk = 32ull << 20;
+    char *buf[2] = {nullptr, nullptr};
+    hipEvent_t done[2] = {nullptr, nullptr};
+    bool busy[2] = {false, false};
+    hipStream_t s = nullptr;
+    int next = 0;
+    void init() {
+        MMCSF_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        for (int i = 0; i < 2; ++i) {
+            MMCSF_HIP(hipHostMalloc(reinterpret_cast<void **>(&buf[i]), kChunk));
+            MMCSF_HIP(hipEventCreateWithFlags(&done[i], hipEventDisableTiming));
+        }
+    }
+    // the next buffer, once the copy that last used it has finished
+    int take() {
+        const int i = next;
+        next ^= 1;
+        if (busy[i]) MMCSF_HIP(hipEventSynchronize(done[i]));
+        busy[i] = false;
+        return i;
+    }
+    void drain() {
+        if (s) MMCSF_HIP(hipStreamSynchronize(s));
+        busy[0] = busy[1] = false;
+    }
+    ~Stage() {
+        if (s) (void)hipStreamSynchronize(s);
+        for (int i = 0; i < 2; ++i) {
+            if (buf[i]) (void)hipHostFree(buf[i]);
+            if (done[i]) (void)hipEventDestroy(done[i]);
+        }
+        if (s) (void)hipStreamDestroy(s);
+    }
+};
+
 struct Writer {
     FILE *f;
+    Stage &st;
     void bytes(const void *p, size_t n) {
         if (n && std::fwrite(p, 1, n, f) != n) fail(MMCSF_ERR_IO, "format cache: write failed");
     }
@@ -35,10 +75,28 @@ struct Writer {
     template <typename T>
     void dev(const DevArray<T> &a) {
         pod<uint64_t>(a.n);
-        if (!a.n) return;
-        std::vector<T> h(a.n);
-        MMCSF_HIP(hipMemcpy(h.data(), a.p, a.n * sizeof(T), hipMemcpyDeviceToHost));
-        bytes(h.data(), a.n * sizeof(T));
+        const size_t total = a.n * sizeof(T);
+        const char *src = reinterpret_cast<const char *>(a.p);
+        const size_t chunks = (total + Stage::kChunk - 1) / Stage::kChunk;
+        int slot[2] = {0, 0};
+        auto issue = [&](size_t c) {
+            const size_t o = c * Stage::kChunk;
+            const int i = st.take();
+            slot[c & 1] = i;
+            MMCSF_HIP(hipMemcpyAsync(st.buf[i], src + o, std::min(Stage::kChunk, total - o), hipMemcpyDeviceToHost,
+                                     st.s));
+            MMCSF_HIP(hipEventRecord(st.done[i], st.s));
+            st.busy[i] = true;
+        };
+        if (chunks) issue(0);
+        for (size_t c = 0; c < chunks; ++c) {
+            const int i = slot[c & 1];
+            MMCSF_HIP(hipEventSynchronize(st.done[i]));
+            st.busy[i] = false;
+            if (c + 1 < chunks) issue(c + 1);  // copies while this chunk is written
+            const size_t o = c * Stage::kChunk;
+            bytes(st.buf[i], std::min(Stage::kChunk, total - o));
+        }
     }
     template <typename T>
     void vec(const std::vector<T> &v) {
@@ -50,6 +108,7 @@ struct Writer {
 struct Reader {
     FILE *f;
     uint64_t left;  // bytes not yet read
+    Stage &st;
     void bytes(void *p, size_t n) {
         MMCSF_REQUIRE(n <= left, MMCSF_ERR_IO, "format cache: file is truncated");
         if (n && std::fread(p, 1, n, f) != n) fail(MMCSF_ERR_IO, "format cache: read failed");
@@ -71,10 +130,17 @@ struct Reader {
         const uint64_t n = count(sizeof(T));
         a.release();
         if (!n) return;
-        std::vector<T> h(n);
-        bytes(h.data(), n * sizeof(T));
         a.alloc(n);
-        MMCSF_HIP(hipMemcpy(a.p, h.data(), n * sizeof(T), hipMemcpyHostToDevice));
+        const size_t total = n * sizeof(T);
+        char *dst = reinterpret_cast<char *>(a.p);
+        for (size_t o = 0; o < total; o += Stage::kChunk) {
+            const size_t m = std::min(Stage::kChunk, total - o);
+            const int i = st.take();
+            bytes(st.buf[i], m);  // reads while the previous chunk copies
+            MMCSF_HIP(hipMemcpyAsync(dst + o, st.buf[i], m, hipMemcpyHostToDevice, st.s));
+            MMCSF_HIP(hipEventRecord(st.done[i], st.s));
+            st.busy[i] = true;
+        }
     }
     template <typename T>
     void vec(std::vector<T> &v) {
@@ -188,7 +254,9 @@ void format_save(const Format &f, const char *path) {
         ~Closer() { std::fclose(f); }
     } closer{fp};
     MMCSF_HIP(hipDeviceSynchronize());
-    Writer w{fp};
+    Stage st;
+    st.init();
+    Writer w{fp, st};
     w.bytes(kMagic, 8);
     w.pod(kVersion);
     w.pod(kByteOrder);
@@ -216,7 +284,9 @@ void format_load(const char *path, Format &f) {
     MMCSF_REQUIRE(std::fseek(fp, 0, SEEK_END) == 0, MMCSF_ERR_IO, "format cache: cannot seek");
     const long size = std::ftell(fp);
     MMCSF_REQUIRE(size >= 0 && std::fseek(fp, 0, SEEK_SET) == 0, MMCSF_ERR_IO, "format cache: cannot seek");
-    Reader r{fp, (uint64_t)size};
+    Stage st;
+    st.init();
+    Reader r{fp, (uint64_t)size, st};
     char m[8];
     r.bytes(m, 8);
     MMCSF_REQUIRE(std::memcmp(m, kMagic, 8) == 0, MMCSF_ERR_IO, "not a format cache file");
@@ -226,6 +296,7 @@ void format_load(const char *path, Format &f) {
                       std::to_string(kVersion));
     MMCSF_REQUIRE(r.pod<uint32_t>() == kByteOrder, MMCSF_ERR_IO, "format cache written with another byte order");
     format_fields(r, f);
+    st.drain();
     r.bytes(m, 8);
     MMCSF_REQUIRE(std::memcmp(m, kEnd, 8) == 0 && r.left == 0, MMCSF_ERR_IO, "format cache: bad end of file");
     MMCSF_REQUIRE(f.ndims == 3 || f.ndims == 4, MMCSF_ERR_IO, "format cache: bad tensor order");

@@ -18,12 +18,14 @@ import numpy as np
 
 __all__ = [
     "lib", "MMCSFError", "Options", "Tensor", "Format", "mttkrp", "mttkrp_allmode",
-    "TYPE_COO_CPU", "TYPE_COO", "TYPE_BCSF", "TYPE_BCSF_ANYMODE", "TYPE_HBCSF", "TYPE_MMCSF", "TYPE_ONECSF",
+    "TYPE_COO_CPU", "TYPE_COO", "TYPE_BCSF", "TYPE_BCSF_ANYMODE", "TYPE_HBCSF", "TYPE_MMCSF", "TYPE_SINGLECSF",
+    "TYPE_ONECSF",
     "F32", "BF16",
     "parse_options", "to_bf16", "cpd_als", "gram",
 ]
 
-TYPE_COO_CPU, TYPE_COO, TYPE_BCSF, TYPE_BCSF_ANYMODE, TYPE_HBCSF, TYPE_MMCSF, TYPE_ONECSF = 1, 3, 8, 9, 10, 12, 14
+TYPE_COO_CPU, TYPE_COO, TYPE_BCSF, TYPE_BCSF_ANYMODE, TYPE_HBCSF, TYPE_MMCSF = 1, 3, 8, 9, 10, 12
+TYPE_SINGLECSF, TYPE_ONECSF = 13, 14
 F32, BF16 = 0, 1
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -57,7 +59,7 @@ class FormatInfo(C.Structure):
         ("droppedSlices", C.c_uint64), ("COOnnz", C.c_uint64), ("CSLnnz", C.c_uint64),
         ("HCSRnnz", C.c_uint64), ("redunMode", C.c_int), ("modeTrees", C.c_int),
         ("modeLevel", (C.c_uint64 * 4) * 4), ("deviceBytes", C.c_uint64), ("partNnz", C.c_uint64 * 4),
-        ("mixedPart", C.c_int),
+        ("mixedPart", C.c_int), ("modeTreeOrder", (C.c_int * 4) * 4),
     ]
 
 
@@ -295,6 +297,7 @@ class Format:
             "modeTrees": i.modeTrees,
             "modeLevel": [list(i.modeLevel[m][: i.ndims]) for m in range(i.ndims)],
             "deviceBytes": i.deviceBytes, "partNnz": list(i.partNnz[: i.ndims]), "mixedPart": i.mixedPart,
+            "modeTreeOrder": [list(i.modeTreeOrder[m][: i.ndims]) for m in range(i.ndims)],
         }
 
     def export(self, name, k=0):
@@ -389,7 +392,7 @@ def mttkrp(fmt, mode, factors, out=None, stream=None):
 
 
 def mttkrp_allmode(fmt, factors, outs=None, stream=None, timed=False):
-    """All-mode MTTKRP of an MM-CSF or one-tree (types 9 / 14) format;
+    """All-mode MTTKRP of an MM-CSF or one-tree (types 9 / 13 / 14) format;
     returns (outs, ms_per_mode or None)."""
     import torch
     R = factors[0].shape[1]

@@ -1,7 +1,7 @@
 // mttkrp -- command-line driver with the reference's flags and output lines
 // (src/mttkrp.cu:71-472), running every GPU type on gfx950 through the C ABI.
 //
-//   mttkrp -i X.tns|X.fmt [-m mode] [-R rank] [-t 1|3|8|9|10|12|14] [-f fbrThreshold]
+//   mttkrp -i X.tns|X.fmt [-m mode] [-R rank] [-t 1|3|8|9|10|12|13|14] [-f fbrThreshold]
 //          [-w w] [-b b] [-s s] [-c 1] [-o out.txt] [-v 1] [-d 0|1] [-M 0|1] [-k X.fmt]
 //
 // -d 1 (new) stores the factors in bfloat16.  -M 1 (new) builds MM-CSF with
@@ -61,6 +61,7 @@ void print_help() {
         "        -R rank/feature : set the rank (default 32)\n"
         "        -m mode : set the mode of MTTKRP (default 0, MMCSF evaluates all modes)\n"
         "        -t implementation type: 1: COO CPU, 3: COO GPU 8: B-CSF 10: HB-CSF 12: MM-CSF on GPU (default 1)\n"
+        "           9: B-CSF, every mode; 13: single CSF, every mode; 14: single CSF with bins, every mode\n"
         "        -f fiber-splitting threshold: maximum nonzeros per fiber (default inf)\n"
         "        -w warp per slice (default 4); -w, -s, -b and -g are checked against the reference's limits\n"
         "           and do not shape the launch (the chunked walk splits work by traversal cost)\n"
@@ -248,15 +249,32 @@ int main(int argc, char **argv) {
         result.resize(dims[resultMode] * R);
         HK(hipMemcpy(result.data(), Od[resultMode], result.size() * 4, hipMemcpyDeviceToHost));
         mmcsf_format_free(F);
-    } else if (type == MMCSF_TYPE_BCSF_ANYMODE || type == MMCSF_TYPE_ONECSF) {
+    } else if (type == MMCSF_TYPE_BCSF_ANYMODE || type == MMCSF_TYPE_ONECSF || type == MMCSF_TYPE_SINGLECSF) {
         // one tree, every mode (mttkrp.cu:223-231, 234-299)
+        if (type == MMCSF_TYPE_SINGLECSF && opt.verbose)
+            std::printf("Starting sameCSF: MTTKRP on all modes using same CSF\n");
         double tb = seconds();
         mmcsf_format *F = make_format(opt);
         if (opt.verbose) std::printf("GPU format build - time: %.3f sec \n", seconds() - tb);
         std::vector<float> ms(ndims, 0.f);
         CK(mmcsf_mttkrp_allmode(F, (int)R, dtype, Ud.data(), Od.data(), nullptr, ms.data()));
-        for (int m = 0; m < ndims; ++m)
-            std::printf("%s-GPU-mode %d :%g,\n", type == MMCSF_TYPE_BCSF_ANYMODE ? "ONE-B-CSF" : "singleCSF", m, ms[m]);
+        if (type == MMCSF_TYPE_SINGLECSF) {
+            // the role lines of MTTKRP_HCSR_CPU / _mode1 / _mode2 (mttkrp_cpu.cpp:171, 235, 289),
+            // then the time of each mode
+            mmcsf_format_info info;
+            CK(mmcsf_format_info_get(F, &info));
+            const int *o = info.modeOrder;
+            for (int m = 0; m < ndims; ++m) {
+                if (ndims == 3 && m == o[0]) std::printf("mode 0: %d %d %d\n", o[0], o[1], o[2]);
+                else if (ndims == 3 && m == o[1]) std::printf("mode 1: %d %d %d\n", m, o[2], o[0]);
+                else if (ndims == 3 && m == o[2]) std::printf("mode 2: %d %d %d\n", m, o[0], o[1]);
+                std::printf("sameCSF-GPU-mode %d :%g,\n", m, ms[m]);
+            }
+        } else {
+            for (int m = 0; m < ndims; ++m)
+                std::printf("%s-GPU-mode %d :%g,\n", type == MMCSF_TYPE_BCSF_ANYMODE ? "ONE-B-CSF" : "singleCSF", m,
+                            ms[m]);
+        }
         resultMode = ndims - 1;
         result.resize(dims[resultMode] * R);
         HK(hipMemcpy(result.data(), Od[resultMode], result.size() * 4, hipMemcpyDeviceToHost));

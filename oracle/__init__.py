@@ -318,7 +318,7 @@ def build_format(fmt_type, dims, inds, vals, mode=0, fbr_threshold=99999999):
     ndims = len(dims)
     nnz = len(vals)
     vals = np.asarray(vals, dtype=np.float32)
-    if fmt_type in (1, 3, 8, 10):
+    if fmt_type in (1, 3, 8, 10, 13):
         order = order_tensormode(dims, mode)
         perm, keys = _sorted_keys(inds, order)
         out = {"type": fmt_type, "modeOrder": order, "perm": perm,
@@ -332,6 +332,11 @@ def build_format(fmt_type, dims, inds, vals, mode=0, fbr_threshold=99999999):
             # create_HCSR (util.h:760-850): no fbrS rule at all
             ptr, idx, nf = build_csf(keys, fbr_threshold, NO_SPLIT)
         out.update(fbrPtr=ptr, fbrIdx=idx, nFibers=nf, leaf=keys[-1])
+        if fmt_type == 13:
+            # sort_COOtensor + create_HCSR + create_fbrLikeSlcInds (mttkrp.cu:244-253,
+            # util.h:1755-1766); no bins
+            out["fbrLikeSlcInds"] = fbr_like_slc_inds(ptr, idx, nf)
+            return out
         if fmt_type == 8:
             out["fbrLikeSlcInds"] = fbr_like_slc_inds(ptr, idx, nf)
             bins, w, dropped = tiled_bin(ndims, nnz, ptr)

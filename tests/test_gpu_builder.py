@@ -54,7 +54,7 @@ def check_format(mmcsf, t, fmt_type, dims, inds, vals, mode, f):
             np.testing.assert_array_equal(fmt.export("inds", m), ref["keys"][ref["modeOrder"].index(m)])
         np.testing.assert_array_equal(fmt.export("vals"), ref["vals"])
         return fmt, ref
-    if fmt_type in (8, 12):
+    if fmt_type in (8, 12, 13):
         check_csf(fmt, ref, N)
         assert info["nFibers"] == ref["nFibers"]
         np.testing.assert_array_equal(fmt.export("fbrLikeSlcInds")[:-1], ref["fbrLikeSlcInds"][:-1])
@@ -99,9 +99,20 @@ def test_toy_all_types(cuda, golden_dir):
     import mmcsf
     t = mmcsf.Tensor.load_tns(f"{golden_dir}/toy.tns")
     dims, inds, vals = O.load_tns(f"{golden_dir}/toy.tns")
-    for fmt_type in (1, 3, 8, 10, 12):
+    for fmt_type in (1, 3, 8, 10, 12, 13):
         for mode in range(3):
             check_format(mmcsf, t, fmt_type, dims, inds, vals, mode, 128)
+    # the tree each mode's MTTKRP traverses: its own root tree (-a 1), or the
+    # MM-CSF tree for every mode (-a 0) and the one tree of -t 13
+    for fm in (mmcsf.Format(t, 12, fbr_threshold=128), mmcsf.Format(t, 12, fbr_threshold=128, mode_trees=False),
+               mmcsf.Format(t, 13, mode=1)):
+        info = fm.info()
+        for m in range(3):
+            if info["modeTrees"]:
+                assert info["modeTreeOrder"][m][0] == m
+                assert sorted(info["modeTreeOrder"][m]) == [0, 1, 2]
+            else:
+                assert info["modeTreeOrder"][m] == info["modeOrder"]
     # the Appendix-B MM-CSF arrays, straight from the device
     fmt = mmcsf.Format(t, 12, fbr_threshold=128)
     assert list(fmt.export("fbrPtr", 0)) == [0, 2, 5, 7]
@@ -130,7 +141,7 @@ def test_random_tensors_all_types(cuda, dims, nnz, skew):
         check_format(mmcsf, t, 12, dims, inds, vals, 0, f)
     for mode in range(N):
         for f in (3, 128):
-            for fmt_type in (3, 8, 10):
+            for fmt_type in (3, 8, 10, 13):
                 check_format(mmcsf, t, fmt_type, dims, inds, vals, mode, f)
 
 
@@ -141,7 +152,7 @@ def test_single_nonzero_and_single_slice(cuda):
                        ([np.zeros(50, np.int64), np.arange(50) % 6, np.arange(50) // 6 % 7],
                         np.linspace(1, 2, 50).astype(np.float32))]:
         t = mmcsf.Tensor.from_numpy(dims, inds, vals)
-        for fmt_type in (3, 8, 10, 12):
+        for fmt_type in (3, 8, 10, 12, 13):
             check_format(mmcsf, t, fmt_type, dims, inds, vals, 0, 4)
 
 

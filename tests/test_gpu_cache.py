@@ -19,7 +19,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 CLI = os.path.join(ROOT, "mm-csf_amd", "bin", "mttkrp")
 
 FORMATS = [(3, {}), (8, {}), (9, {}), (10, {}), (12, {}), (12, {"mode_trees": False}), (12, {"mixed": True}),
-           (14, {})]
+           (13, {}), (14, {})]
 
 
 def rel(a, b):
@@ -51,7 +51,7 @@ def test_cache_round_trip_bit_exact(cuda, tmp_path, dims, skew, fmt_type, kw):
     R = 32
     U = [rng.random((d, R)).astype(np.float32) for d in dims]
     Ud = [torch.from_numpy(u).to(cuda) for u in U]
-    modes = range(len(dims)) if fmt_type in (9, 12, 14) else [i1["modeOrder"][0]]
+    modes = range(len(dims)) if fmt_type in (9, 12, 13, 14) else [i1["modeOrder"][0]]
     for m in modes:
         got = mmcsf.mttkrp(f3, m, Ud).cpu().numpy()
         assert rel(got, O.mttkrp_f64_mt(inds, vals, m, U)) < 1e-5, f"mode {m}"

@@ -120,6 +120,29 @@ def test_bf16_factor_storage(cuda, dims, nnz, skew, R):
         assert rel(got, O.mttkrp_coo_f64(inds, vals, 1, Ur)) < 1e-5
 
 
+@pytest.mark.parametrize("dims,nnz,skew", [CASES[1], CASES[4]])
+@pytest.mark.parametrize("R", [12, 20, 128, 256])
+def test_bf16_odd_and_wide_ranks(cuda, dims, nnz, skew, R):
+    """bfloat16 rows of 8 bytes per lane (R % 8 == 4: R = 12, 20) and of two
+    or four column blocks (R = 128, 256), on a skewed 3-D and a skewed 4-D
+    tensor, with and without mode trees."""
+    import torch
+    import mmcsf
+    rng = np.random.default_rng(11 * nnz + R)
+    inds, vals = random_tensor(rng, dims, nnz, skew)
+    U, Ud = factors(rng, dims, R, torch, cuda)
+    Ub = [u.to(torch.bfloat16) for u in Ud]
+    Ur = bf16_round(U, torch)
+    t = mmcsf.Tensor.from_numpy(dims, inds, vals)
+    for trees in (True, False):
+        fm = mmcsf.Format(t, 12, fbr_threshold=128, mode_trees=trees)
+        outs, _ = mmcsf.mttkrp_allmode(fm, Ub)
+        for m in range(len(dims)):
+            got = outs[m].cpu().numpy()
+            assert rel(got, O.mttkrp_f64_mt(inds, vals, m, Ur)) < 1e-5, f"R {R} trees {trees} mode {m}"
+            assert rel(got, O.mttkrp_f64_mt(inds, vals, m, U)) < 1e-3
+
+
 def test_uniform_2000_cubed_vs_t1_reference(cuda):
     """The plain mode-0 workload: uniform 2000^3, 1M distinct nonzeros, R=32,
     -f 128; GPU COO and B-CSF (and MM-CSF) against the -t 1 sequential float32
@@ -210,10 +233,12 @@ def test_error_paths(cuda):
 
 @pytest.mark.parametrize("dims,skew", [([60, 50, 40], None), ([300, 200, 100], [1.2, 1.2, 0]),
                                        ([30, 20, 25, 15], None), ([200, 40, 150, 8], [1.1, 0, 1.1, 0])])
-@pytest.mark.parametrize("fmt_type", [9, 14])
+@pytest.mark.parametrize("fmt_type", [9, 13, 14])
 def test_one_tree_types_every_mode(cuda, dims, skew, fmt_type):
-    """-t 9 / -t 14: the B-CSF arrays of -t 8, every mode computed from that one
-    tree (root walk for modeOrder[0], scatter roles for the other levels)."""
+    """-t 9 / -t 13 / -t 14: the B-CSF arrays of -t 8 (type 13 without the slice
+    bins), every mode computed from that one tree (root walk for modeOrder[0],
+    scatter roles for the other levels).  For 4-D, type 13 computes every mode
+    exactly where the reference's host roles cover the root mode only."""
     import torch
     import mmcsf
     rng = np.random.default_rng(fmt_type * 7 + len(dims))
@@ -226,8 +251,13 @@ def test_one_tree_types_every_mode(cuda, dims, skew, fmt_type):
         for l in range(len(dims) - 1):
             np.testing.assert_array_equal(fo.export("fbrPtr", l), f8.export("fbrPtr", l))
             np.testing.assert_array_equal(fo.export("fbrIdx", l), f8.export("fbrIdx", l))
-        for b in range(10):
-            np.testing.assert_array_equal(fo.export("slcMapperBin", b), f8.export("slcMapperBin", b))
+        if fmt_type == 13:
+            with pytest.raises(mmcsf.MMCSFError, match="no such bin"):
+                fo.export("slcMapperBin", 0)
+            np.testing.assert_array_equal(fo.export("fbrLikeSlcInds")[:-1], f8.export("fbrLikeSlcInds")[:-1])
+        else:
+            for b in range(10):
+                np.testing.assert_array_equal(fo.export("slcMapperBin", b), f8.export("slcMapperBin", b))
         for R in (4, 32, 33):
             U, Ud = factors(rng, dims, R, torch, cuda)
             outs, _ = mmcsf.mttkrp_allmode(fo, Ud)
@@ -245,7 +275,7 @@ def test_one_tree_types_every_mode(cuda, dims, skew, fmt_type):
             assert rel(outs[m].cpu().numpy(), O.mttkrp_coo_f64(inds, vals, m, Ur)) < 1e-5
 
 
-@pytest.mark.parametrize("fmt_type", [9, 14])
+@pytest.mark.parametrize("fmt_type", [9, 13, 14])
 def test_one_tree_cli(cuda, golden_dir, fmt_type):
     import os
     import subprocess
@@ -254,14 +284,18 @@ def test_one_tree_cli(cuda, golden_dir, fmt_type):
     r = subprocess.run([cli, "-i", f"{golden_dir}/toy.tns", "-t", str(fmt_type), "-R", "4", "-c", "1"],
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
-    tag = "ONE-B-CSF-GPU-mode" if fmt_type == 9 else "singleCSF-GPU-mode"
+    tag = {9: "ONE-B-CSF-GPU-mode", 13: "sameCSF-GPU-mode", 14: "singleCSF-GPU-mode"}[fmt_type]
     assert r.stdout.count(tag) == 3
+    if fmt_type == 13:
+        # the role lines of MTTKRP_HCSR_CPU / _mode1 / _mode2 for the toy's order 0-1-2
+        for line in ("mode 0: 0 1 2", "mode 1: 1 2 0", "mode 2: 2 0 1"):
+            assert line in r.stdout
     assert "Correctness pass!" in r.stdout
 
 
 @pytest.mark.parametrize("dims,skew", [([60, 50, 40], None), ([300, 200, 100], [1.2, 1.2, 0]),
                                        ([30, 20, 25, 15], None)])
-@pytest.mark.parametrize("fmt_type", [8, 14])
+@pytest.mark.parametrize("fmt_type", [8, 13, 14])
 def test_ttm_matches_float64_and_host(cuda, dims, skew, fmt_type):
     """TTM per leaf fiber (TTM_GPU) against the float64 restatement of TTM_CPU
     (oracle/dense.py) on the exported tree, and against the host TTM."""
@@ -310,7 +344,8 @@ def test_ttm_cli(cuda, golden_dir, tmp_path):
 
 
 @pytest.mark.parametrize("fmt_type,tag", [(1, "COO CPU - time"), (3, "Loop on. COO GPU using loop - time"),
-                                          (8, "B-CSF-GPU-mode"), (10, "HYB GPU:"), (12, "Total GPU time:")])
+                                          (8, "B-CSF-GPU-mode"), (10, "HYB GPU:"), (12, "Total GPU time:"),
+                                          (13, "sameCSF-GPU-mode 2")])
 def test_mttkrp_cli_every_type(cuda, golden_dir, tmp_path, fmt_type, tag):
     """The driver's stdout lines (mttkrp.cu), -c 1 against the COO CPU type and
     the -o matrix file (write_output, util.h:3270-3283)."""

@@ -187,6 +187,23 @@ def test_toy_bcsf_arrays(golden_dir):
     assert b["dropped"] == 0
 
 
+def test_toy_single_csf_arrays(golden_dir):
+    # -t 13 on the toy at -m 0: the hand-derived create_HCSR tree of the B-CSF
+    # test above, fbrLikeSlcInds = the slice of each fiber, no bins
+    dims, inds, vals = O.load_tns(f"{golden_dir}/toy.tns")
+    s = O.build_format(13, dims, inds, vals, mode=0, fbr_threshold=128)
+    assert s["modeOrder"] == [0, 1, 2]
+    assert list(s["fbrPtr"][0]) == [0, 3, 5, 7]
+    assert list(s["fbrPtr"][1]) == [0, 1, 2, 3, 4, 6, 7, 8]
+    assert list(s["fbrIdx"][1][:-1]) == [0, 1, 2, 0, 2, 0, 1]
+    assert list(s["fbrLikeSlcInds"][:-1]) == [0, 0, 0, 1, 1, 2, 2]
+    assert "slcMapperBin" not in s
+    # -f 1 splits the two-nonzero fiber (slice 1, index 2) in two
+    s1 = O.build_format(13, dims, inds, vals, mode=0, fbr_threshold=1)
+    assert list(s1["fbrPtr"][1]) == [0, 1, 2, 3, 4, 5, 6, 7, 8]
+    assert list(s1["fbrPtr"][0]) == [0, 3, 6, 8]
+
+
 def test_toy_hyb_arrays(golden_dir):
     # -m 0: slices 0 and 2 have only length-1 fibers (CSL), slice 1 is CSF
     dims, inds, vals = O.load_tns(f"{golden_dir}/toy.tns")
