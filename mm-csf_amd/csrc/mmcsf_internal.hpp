@@ -221,6 +221,9 @@ void mttkrp_mode(const Format &f, int mode, const Factors &F, float *out, hipStr
 // the start bitmasks of the tree
 bool walk_supports(const Csf &c, int R);
 void mttkrp_root(const Csf &c, const Factors &F, float *out, hipStream_t s, bool accumulate = false);
+// the walk's row loads are 16 bytes wide (8 for bfloat16 with R % 8 != 0):
+// factor tables of every level but skipLevel, and `out`, must be aligned to that
+void check_walk_align(const Csf &c, const Factors &F, int skipLevel, const float *out);
 void mttkrp_hyb(const Hyb &h, const uint64_t *dims, const Factors &F, float *out, hipStream_t s);
 // loads the code objects of the MTTKRP kernels (and, cp, of the CP
 // completion passes) on the current device, once per process

@@ -615,13 +615,16 @@ void mttkrp_coo(const Format &f, const Factors &F, float *out, hipStream_t s) {
 
 void mttkrp_hyb(const Hyb &h, const uint64_t *dims, const Factors &F, float *out, hipStream_t s) {
     const int N = h.ndims;
-    fill_zero(out, dims[h.order[0]] * (uint64_t)F.R, s);
     Geo g = pick_geometry(F.R, F.dtype);
     check_align(out, MMCSF_F32, g);
     for (int l = 1; l < N; ++l) {
         MMCSF_REQUIRE(F.U[h.order[l]] != nullptr, MMCSF_ERR_ARG, "missing factor matrix");
         check_align(F.U[h.order[l]], F.dtype, g);
     }
+    // the CSF part's walk loads wider rows than the COO / CSL kernels: refuse
+    // its misaligned factors before any part is launched
+    if (h.csf.nnz && walk_supports(h.csf, F.R)) check_walk_align(h.csf, F, 0, out);
+    fill_zero(out, dims[h.order[0]] * (uint64_t)F.R, s);
     // COO part: slices with one nonzero; each output row is written once
     if (h.cooNnz) {
         CooArgs a{};

@@ -57,7 +57,7 @@ static bool tables_fit(const Csf &c, int R) {
 
 // row loads of the walk are VEC elements wide (16 bytes, or 8 bytes for
 // bfloat16 rows with R % 8 != 0); rows of `out` are written as float4
-static void check_walk_align(const Csf &c, const Factors &F, int skipLevel, const float *out) {
+void check_walk_align(const Csf &c, const Factors &F, int skipLevel, const float *out) {
     const uintptr_t need = (F.dtype == MMCSF_BF16 && F.R % 8 != 0) ? 8 : 16;
     for (int l = 0; l < c.ndims; ++l) {
         if (l == skipLevel) continue;

@@ -229,6 +229,24 @@ def test_error_paths(cuda):
     mis = big[1:].view(10, 8)
     with pytest.raises(mmcsf.MMCSFError):
         mmcsf.mttkrp(fm, 1, [mis, None, Ud[2]])
+    # HB-CSF with bfloat16 rows of R = 8 (16-byte row loads in the CSF part's
+    # walk): factors 8 bytes off a 16-byte boundary are refused before any
+    # part runs, and the output is left untouched
+    fh = mmcsf.Format(t, 10, mode=0)
+    assert fh.info()["HCSRnnz"] > 0
+    Ub = [torch.rand((d, 8), device=cuda).to(torch.bfloat16) for d in dims]
+    for m in (1, 2):
+        raw = torch.empty(dims[m] * 8 + 4, dtype=torch.bfloat16, device=cuda)
+        off = raw[4:].view(dims[m], 8)
+        off.copy_(Ub[m])
+        assert off.data_ptr() % 16 == 8
+        bad = list(Ub)
+        bad[m] = off
+        out = torch.full((dims[0], 8), 7.0, device=cuda)
+        with pytest.raises(mmcsf.MMCSFError, match="aligned to 16"):
+            mmcsf.mttkrp(fh, 0, bad, out)
+        torch.cuda.synchronize()
+        assert bool((out == 7.0).all())
 
 
 @pytest.mark.parametrize("dims,skew", [([60, 50, 40], None), ([300, 200, 100], [1.2, 1.2, 0]),
