@@ -5,8 +5,8 @@
 //   MTTKRP                 the MM-CSF kernels (float32 accumulation)
 //   Gram  A^T A            MFMA v_mfma_f32_16x16x4f32 over row chunks, chunk sums
 //                          reduced in float64
-//   Hadamard of Grams, the R x R inverse (Gauss-Jordan in LDS for R <= 36,
-//   Cholesky above; LU with partial pivoting when the matrix is not SPD),
+//   Hadamard of Grams, the R x R inverse (Cholesky, in LDS for R <= 36; LU
+//   with partial pivoting when the matrix is not SPD),
 //   the row solve U = M H^-1, column norms and the fit     float64
 #include <algorithm>
 #include <chrono>
@@ -102,10 +102,10 @@ __global__ void k_zero_d(double *p, uint64_t n) {
 
 // ---------------------------------------------------------------------------
 // H = Hadamard_{n != skip} G[n]; Hinv = H^-1, or by LU with partial pivoting
-// when a pivot is not positive (MatrixSolveNormals, cpd_cpu.cu:140-208).  One
-// workgroup; W is an R*R*3 float64 workspace.  Small R (the matrices in LDS):
-// Gauss-Jordan elimination without pivoting, all R*R entries updated in
-// parallel per step; larger R: Cholesky (lower) and the inverse of L.
+// when the Cholesky factorization fails (MatrixSolveNormals,
+// cpd_cpu.cu:140-208).  One workgroup; W is an R*R*3 float64 workspace, and
+// for R <= 36 the three matrices live in LDS instead.  Cholesky (lower), the
+// inverse X of L, then H^-1 = X^T X.
 // ---------------------------------------------------------------------------
 __global__ void k_normal_inverse(const double *__restrict__ G, int N, int skip, int R, double *__restrict__ W,
                                  double *__restrict__ Hinv, int *__restrict__ status) {
@@ -126,50 +126,29 @@ __global__ void k_normal_inverse(const double *__restrict__ G, int N, int skip, 
     }
     if (tid == 0) fail = 0;
     __syncthreads();
-    if (inLds) {
-        constexpr int kPer = 6;  // entries per thread: R <= 36 here
-        bool ok = true;
-        for (int k = 0; k < R; ++k) {
-            const double p = H[(size_t)k * R + k];  // the same value in every thread
-            ok = p > 0.0;
-            if (!ok) break;
-            double nv[kPer];
-            int q = 0;
-            for (int e = tid; e < R * R; e += nt, ++q) {
-                const int i = e / R, j = e % R;
-                const double hik = H[(size_t)i * R + k], hkj = H[(size_t)k * R + j];
-                nv[q] = i == k ? (j == k ? 1.0 / p : hkj / p) : (j == k ? -hik / p : H[e] - hik * hkj / p);
-            }
-            __syncthreads();
-            q = 0;
-            for (int e = tid; e < R * R; e += nt, ++q) H[e] = nv[q];
-            __syncthreads();
+    // Cholesky H = L L^T (lower; spotrf, cpd_cpu.cu:168-179), right-looking: per
+    // column k the diagonal, the column below it, then the trailing lower
+    // triangle, each step's entries updated in parallel.  A diagonal that is
+    // not positive (the leading minor is not positive definite) sends the
+    // solve to LU, as spotrf's failure sends the reference to sgesv.
+    for (int k = 0; k < R; ++k) {
+        const double d = H[(size_t)k * R + k];  // the same value in every thread
+        if (!(d > 0.0)) {
+            if (tid == 0) fail = 1;
+            break;
         }
-        if (ok) {
-            for (int e = tid; e < R * R; e += nt) Hinv[e] = H[e];
-            if (tid == 0) status[0] = 0;
-            return;
-        }
-        if (tid == 0) fail = 1;
+        const double lkk = sqrt(d);
+        for (int i = k + 1 + tid; i < R; i += nt) L[(size_t)i * R + k] = H[(size_t)i * R + k] / lkk;
+        if (tid == 0) L[(size_t)k * R + k] = lkk;
         __syncthreads();
-    }
-    // Cholesky, column by column
-    for (int j = 0; j < R && !inLds; ++j) {
-        if (tid == 0) {
-            double d = H[(size_t)j * R + j];
-            for (int k = 0; k < j; ++k) d -= L[(size_t)j * R + k] * L[(size_t)j * R + k];
-            if (!(d > 0.0)) fail = 1;
-            L[(size_t)j * R + j] = d > 0.0 ? sqrt(d) : 1.0;
-        }
-        __syncthreads();
-        if (fail) break;
-        for (int i = j + 1 + tid; i < R; i += nt) {
-            double s = H[(size_t)i * R + j];
-            for (int k = 0; k < j; ++k) s -= L[(size_t)i * R + k] * L[(size_t)j * R + k];
-            L[(size_t)i * R + j] = s / L[(size_t)j * R + j];
+        const int m = R - k - 1;
+        for (int e = tid; e < m * m; e += nt) {
+            const int i = k + 1 + e / m, j = k + 1 + e % m;
+            if (j <= i) H[(size_t)i * R + j] -= L[(size_t)i * R + k] * L[(size_t)j * R + k];
         }
         __syncthreads();
     }
+    __syncthreads();
     if (!fail) {
         // X = L^-1 (lower), one column per thread: L x = e_c
         for (int c = tid; c < R; c += nt) {

@@ -41,8 +41,8 @@ def planted_tensor(rng, dims, nnz, rank, zipf0=1.2, noise=0.01):
     return inds, vals.astype(np.float32)
 
 
-# R <= 36 inverts the normal matrix by Gauss-Jordan in LDS, R = 48 by the
-# Cholesky path; R = 5 has a partial column quad in the row solve
+# R <= 36 factors the normal matrix by Cholesky in LDS, R = 48 in the global
+# work space; R = 5 has a partial column quad in the row solve
 @pytest.mark.parametrize("R", [5, 8, 16, 48])
 def test_cpd_als_matches_float64_als(cuda, R):
     import torch
@@ -66,6 +66,26 @@ def test_cpd_als_matches_float64_als(cuda, R):
     mg = D.cp_model_values(inds, [u.cpu().double().numpy() * (lam if m == 0 else 1.0) for m, u in enumerate(Ud)])
     m64 = D.cp_model_values(inds, [u * (lam64 if m == 0 else 1.0) for m, u in enumerate(U64)])
     assert np.linalg.norm(mg - m64) / np.linalg.norm(m64) < 1e-3
+
+
+def test_cpd_als_singular_normal_equations(cuda):
+    """A zero column in a factor makes every normal matrix that multiplies its
+    Gram singular: the Cholesky factorization stops at that column's zero
+    pivot, LU with partial pivoting finds no pivot either, and the call fails
+    with an error instead of returning a model."""
+    import torch
+    import mmcsf
+    rng = np.random.default_rng(3)
+    dims = [300, 200, 100]
+    R = 8
+    inds, vals = planted_tensor(rng, dims, 20000, R)
+    t = mmcsf.Tensor.from_numpy(dims, inds, vals)
+    fmt = mmcsf.Format(t, mmcsf.TYPE_MMCSF, fbr_threshold=128)
+    U0 = [rng.random((d, R)).astype(np.float32) + 0.1 for d in dims]
+    U0[1][:, 3] = 0.0
+    Ud = [torch.from_numpy(u.copy()).to(cuda) for u in U0]
+    with pytest.raises(mmcsf.MMCSFError, match="singular"):
+        mmcsf.cpd_als(fmt, Ud, iters=2, tol=0.0)
 
 
 def test_cpd_als_bf16_mttkrp(cuda):
