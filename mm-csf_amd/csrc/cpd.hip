@@ -150,15 +150,22 @@ __global__ void k_normal_inverse(const double *__restrict__ G, int N, int skip, 
     }
     __syncthreads();
     if (!fail) {
-        // X = L^-1 (lower), one column per thread: L x = e_c
-        for (int c = tid; c < R; c += nt) {
-            for (int i = 0; i < R; ++i) {
-                double s = (i == c) ? 1.0 : 0.0;
-                for (int k = c; k < i; ++k) s -= L[(size_t)i * R + k] * X[(size_t)k * R + c];
-                X[(size_t)i * R + c] = i < c ? 0.0 : s / L[(size_t)i * R + i];
-            }
-        }
+        // X = L^-1 (lower) by column-oriented forward substitution on X = I:
+        // per row k, scale it by 1 / L[k][k], then eliminate column k from the
+        // rows below; every step's entries are updated in parallel
+        for (int e = tid; e < R * R; e += nt) X[e] = (e / R == e % R) ? 1.0 : 0.0;
         __syncthreads();
+        for (int k = 0; k < R; ++k) {
+            const double rk = 1.0 / L[(size_t)k * R + k];
+            for (int c = tid; c <= k; c += nt) X[(size_t)k * R + c] *= rk;
+            __syncthreads();
+            const int m = R - k - 1;
+            for (int e = tid; e < m * (k + 1); e += nt) {
+                const int i = k + 1 + e / (k + 1), c = e % (k + 1);
+                X[(size_t)i * R + c] -= L[(size_t)i * R + k] * X[(size_t)k * R + c];
+            }
+            __syncthreads();
+        }
         // H^-1 = X^T X
         for (int e = tid; e < R * R; e += nt) {
             const int i = e / R, j = e % R;
