@@ -156,9 +156,14 @@ constexpr uint32_t kOobOffset = 0xFFFFF000u;
 // Fiber-level factoring: the rows of the levels between the root and the leaf
 // are read once per fiber segment of a group run instead of once per nonzero.
 // Off for wide ranks (the per-block partials would not fit the registers),
-// for one-lane groups (a run of one nonzero) and for TTM (no upper levels).
-template <int G, int NB, int OP>
-constexpr bool kFactorOf = OP != kTtm && NB <= 2 && G >= 2;
+// for one-lane groups (a run of one nonzero), for TTM (no upper levels) and
+// for 4-D MTTKRP with 8-lane groups: on a skewed 4-D tensor (100000 x 50000 x
+// 20000 x 1000, 50M nonzeros) the leaf fibers hold 1-1.5 nonzeros, so
+// factoring saves few loads while its per-step bookkeeping costs more than at
+// narrower groups (all modes at R=64: bfloat16 3.79 ms unfactored against 4.15
+// factored, float32 7.62 against 7.71; at R=16 factoring still wins).
+template <int N, int G, int NB, int OP>
+constexpr bool kFactorOf = OP != kTtm && NB <= 2 && G >= 2 && !(OP == kMttkrp && N == 4 && G == 8);
 
 template <int N, int G, int NB, int TB, typename T, int VEC, int OP>
 __global__ __launch_bounds__(256, (OP == kMttkrp && N == 3 && NB == 1 && G * VEC <= 32) ? 3 : 2) void k_walk(WalkArgs a) {
@@ -170,7 +175,7 @@ __global__ __launch_bounds__(256, (OP == kMttkrp && N == 3 && NB == 1 && G * VEC
     // multiplies the rows of the upper levels; rows of `out` are keyed by
     // slice (by leaf fiber for TTM, which has no upper levels)
     constexpr bool kUpper = OP != kTtm;
-    constexpr bool kFactor = kFactorOf<G, NB, OP>;
+    constexpr bool kFactor = kFactorOf<N, G, NB, OP>;
     // residuals / weights are addressed by tensor-order position
     constexpr bool kPos = OP == kLossResid || OP == kWeightGrad;
     constexpr int GPR = 64 / G;  // groups (runs) per chunk
