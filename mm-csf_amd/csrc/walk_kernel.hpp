@@ -153,17 +153,20 @@ __device__ __forceinline__ float as_float(uint32_t u) { return __uint_as_float(u
 // kOobOffset bytes, and a column offset below 4 KiB keeps it past them).
 constexpr uint32_t kOobOffset = 0xFFFFF000u;
 
-// Fiber-level factoring: the rows of the levels between the root and the leaf
-// are read once per fiber segment of a group run instead of once per nonzero.
-// Off for wide ranks (the per-block partials would not fit the registers),
-// for one-lane groups (a run of one nonzero), for TTM (no upper levels) and
-// for 4-D MTTKRP with 8-lane groups: on a skewed 4-D tensor (100000 x 50000 x
-// 20000 x 1000, 50M nonzeros) the leaf fibers hold 1-1.5 nonzeros, so
-// factoring saves few loads while its per-step bookkeeping costs more than at
-// narrower groups (all modes at R=64: bfloat16 3.79 ms unfactored against 4.15
-// factored, float32 7.62 against 7.71; at R=16 factoring still wins).
+// Fiber-level factoring (MTTKRP): the rows of the levels between the root and
+// the leaf are read once per fiber segment of a group run instead of once per
+// nonzero.  Off for wide ranks (the per-block partials would not fit the
+// registers), for one-lane groups (a run of one nonzero) and for 4-D trees
+// with 8-lane groups: on a skewed 4-D tensor (100000 x 50000 x 20000 x 1000,
+// 50M nonzeros) the leaf fibers hold 1-1.5 nonzeros, so factoring saves few
+// loads while its per-step bookkeeping costs more than at narrower groups
+// (all modes at R=64: bfloat16 3.79 ms unfactored against 4.15 factored,
+// float32 7.62 against 7.71; at R=16 factoring still wins).  The CP passes
+// multiply per nonzero: they also read the root row and reduce a dot product
+// per nonzero, and on a 1000000 x 500000 x 100000 tensor with 50M nonzeros
+// factoring them gained nothing at R=16 and cost 2-13 % at R=64.
 template <int N, int G, int NB, int OP>
-constexpr bool kFactorOf = OP != kTtm && NB <= 2 && G >= 2 && !(OP == kMttkrp && N == 4 && G == 8);
+constexpr bool kFactorOf = OP == kMttkrp && NB <= 2 && G >= 2 && !(N == 4 && G == 8);
 
 template <int N, int G, int NB, int TB, typename T, int VEC, int OP>
 __global__ __launch_bounds__(256, (OP == kMttkrp && N == 3 && NB == 1 && G * VEC <= 32) ? 3 : 2) void k_walk(WalkArgs a) {
