@@ -20,7 +20,9 @@
 //     fiber and multiplied by the fiber's upper-level rows once, which are
 //     read only where a fiber segment of the run begins (out-of-range buffer
 //     loads for the other lanes, so every load stays unconditional);
-//     CP: the model value <U_root[i], U_1[j] (. U_2[k]) (.) U_leaf[l]> is a
+//     factoring is off for 4-D trees with 8-lane groups (leaf fibers of
+//     1-1.5 nonzeros); CP, per nonzero: the model value
+//     <U_root[i], U_1[j] (. U_2[k]) (.) U_leaf[l]> is a
 //     dot product reduced over the group, e = model - v, the loss sums e^2
 //     in float64, the gradient contribution is alpha * e * (the MTTKRP
 //     product); TTM: v * U_leaf[leaf] summed per leaf fiber;

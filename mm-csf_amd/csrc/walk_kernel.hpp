@@ -179,6 +179,7 @@ __global__ __launch_bounds__(256, (OP == kMttkrp && N == 3 && NB == 1 && G * VEC
     // slice (by leaf fiber for TTM, which has no upper levels)
     constexpr bool kUpper = OP != kTtm;
     constexpr bool kFactor = kFactorOf<N, G, NB, OP>;
+    static_assert(!(kFactor && (OP == kLossGrad || OP == kLossResid)), "the CP passes multiply per nonzero");
     // residuals / weights are addressed by tensor-order position
     constexpr bool kPos = OP == kLossResid || OP == kWeightGrad;
     constexpr int GPR = 64 / G;  // groups (runs) per chunk
@@ -220,7 +221,6 @@ __global__ __launch_bounds__(256, (OP == kMttkrp && N == 3 && NB == 1 && G * VEC
     const T *U1 = static_cast<const T *>(a.U[1]);
     const T *U2 = static_cast<const T *>(a.U[2]);
     const T *UL = static_cast<const T *>(a.U[N - 1]);
-    [[maybe_unused]] const __amdgpu_buffer_rsrc_t rs0 = table_rsrc(a.U[0], a.tabBytes[0]);
     [[maybe_unused]] const __amdgpu_buffer_rsrc_t rs1 = table_rsrc(a.U[1], a.tabBytes[1]);
     [[maybe_unused]] const __amdgpu_buffer_rsrc_t rs2 = table_rsrc(a.U[2], a.tabBytes[2]);
     const uint32_t *__restrict__ ptrL = a.ptr[LF];
@@ -337,7 +337,7 @@ __global__ __launch_bounds__(256, (OP == kMttkrp && N == 3 && NB == 1 && G * VEC
     // level-2 indices of its last nonzero, and (fiber-level factoring) the
     // level-1 / level-2 / root rows in effect there.
     uint32_t i1c = kNone, i2c = kNone;
-    [[maybe_unused]] Raw<T, VEC> cw0[NB], cw1[NB], cw2[NB];
+    [[maybe_unused]] Raw<T, VEC> cw1[NB], cw2[NB];
 
     // one chunk: wc / ec hold chunk c; the entries of chunk c+1 (words wn)
     // go to en, the words of chunk c+2 to wf.  Group g takes the run of
@@ -375,9 +375,6 @@ __global__ __launch_bounds__(256, (OP == kMttkrp && N == 3 && NB == 1 && G * VEC
             FM = __ballot(newF);
             JM = N == 4 ? __ballot(newJ) : FM;
         }
-        // root-row segments (CP model value): slices; before the first chunk
-        // of the run no root row is carried
-        const uint64_t RM = bm | (i1c == kNone ? 1ull : 0ull);
         const uint32_t gs = (uint32_t)(g * STEPS);
         // start bits of the group's steps (bit t: step t)
         const uint32_t gF = (uint32_t)(FM >> gs);
@@ -386,7 +383,6 @@ __global__ __launch_bounds__(256, (OP == kMttkrp && N == 3 && NB == 1 && G * VEC
         // the group's first entry still lies in the segment carried over
         const bool carryF = (FM & low_bits(gs + 1)) == 0;
         const bool carryJ = (JM & low_bits(gs + 1)) == 0;
-        const bool carryR = (RM & low_bits(gs + 1)) == 0;
         // steps at which some group reads rows (wave-uniform; bit t: step t)
         auto fold = [&](uint64_t m, uint64_t m0) -> uint32_t {
             uint32_t r = 0;
@@ -397,7 +393,6 @@ __global__ __launch_bounds__(256, (OP == kMttkrp && N == 3 && NB == 1 && G * VEC
         };
         const uint32_t need1 = kFactor ? fold(N == 3 ? FM : JM, N == 3 ? FM : JM) : 0u;
         const uint32_t need2 = kFactor && N == 4 ? fold(FM, FM) : 0u;
-        const uint32_t need0 = kFactor && kModel ? fold(RM, RM) : 0u;
         if constexpr (kFactor) {
             // the carried rows sit in the last group's lanes: spread them when
             // some run begins inside a carried segment
@@ -412,12 +407,6 @@ __global__ __launch_bounds__(256, (OP == kMttkrp && N == 3 && NB == 1 && G * VEC
                     for (int b = 0; b < NB; ++b) cw2[b].from_lane(cw2[b], last + k);
                 }
             }
-            if constexpr (kModel) {
-                if (!(RM & 1ull)) {
-#pragma unroll
-                    for (int b = 0; b < NB; ++b) cw0[b].from_lane(cw0[b], last + k);
-                }
-            }
         }
         // per-lane offsets and values, taken by the group's lanes step by step;
         // rows a lane does not need read as out-of-range (zero) buffer loads
@@ -428,18 +417,17 @@ __global__ __launch_bounds__(256, (OP == kMttkrp && N == 3 && NB == 1 && G * VEC
                                 ? ec.i1 * rowBytes : kOobOffset;
         const uint32_t o2 = !kFactor ? ec.i2 * rowBytes
                             : ((runStart && !carryF) || (!runStart && (FM >> lane) & 1ull)) ? ec.i2 * rowBytes : kOobOffset;
-        const uint32_t o0 = (!kFactor || (runStart && !carryR) || (!runStart && sstart)) ? (valid ? s : 0u) * rowBytes
-                                                                                         : kOobOffset;
+        const uint32_t o0 = (valid ? s : 0u) * rowBytes;  // root row (CP model value)
 
         // Partial sums of the group run (per column block):
         //   fa  leaf-fiber partial  sum co * U_leaf[leaf] over the open leaf fiber
         //   ga  (4-D) level-1 partial  sum fa (.) U_2[fiber] over its closed fibers
         //   la  slice partial  (products of closed segments with their rows)
         //   hd  partial of the slice open at the run start, once a later slice starts
-        // fr1 / fr2 / fr0: the level-1 / level-2 / root rows of the open
-        // segments (fiber-level factoring).
+        // fr1 / fr2: the level-1 / level-2 rows of the open segments
+        // (fiber-level factoring).
         float hd[NB][VEC], la[NB][VEC], fa[NB][VEC], ga[NB][VEC];
-        Raw<T, VEC> fr0[NB], fr1[NB], fr2[NB];
+        Raw<T, VEC> fr1[NB], fr2[NB];
 #pragma unroll
         for (int b = 0; b < NB; ++b)
 #pragma unroll
@@ -478,7 +466,7 @@ __global__ __launch_bounds__(256, (OP == kMttkrp && N == 3 && NB == 1 && G * VEC
                         w1[t][b].load(U1, l1 + cb4);
                         if constexpr (N == 4) w2[t][b].load(U2, l2 + cb4);
                     }
-                    if constexpr (kModel && !kFactor) wz[t][b].load(U0, l0 + cb4);
+                    if constexpr (kModel) wz[t][b].load(U0, l0 + cb4);
                 }
                 if constexpr (kFactor) {
                     // upper-level rows only at steps where some group opens a
@@ -491,12 +479,6 @@ __global__ __launch_bounds__(256, (OP == kMttkrp && N == 3 && NB == 1 && G * VEC
                         if ((need2 >> tt) & 1u) {
 #pragma unroll
                             for (int b = 0; b < NB; ++b) w2[t][b].loadb(rs2, l2 + colBytes[b]);
-                        }
-                    }
-                    if constexpr (kModel) {
-                        if ((need0 >> tt) & 1u) {
-#pragma unroll
-                            for (int b = 0; b < NB; ++b) wz[t][b].loadb(rs0, l0 + colBytes[b]);
                         }
                     }
                 }
@@ -532,11 +514,6 @@ __global__ __launch_bounds__(256, (OP == kMttkrp && N == 3 && NB == 1 && G * VEC
                                 fr2[b] = cw2[b];
                                 if (need2 & 1u)
                                     if (!carryF) fr2[b] = w2[t][b];
-                            }
-                            if constexpr (kModel) {
-                                fr0[b] = cw0[b];
-                                if (need0 & 1u)
-                                    if (!carryR) fr0[b] = wz[t][b];
                             }
                         }
                     } else if ((gF >> tt) & 1u) {
@@ -577,8 +554,6 @@ __global__ __launch_bounds__(256, (OP == kMttkrp && N == 3 && NB == 1 && G * VEC
                                 }
                                 fr2[b] = w2[t][b];
                             }
-                            if constexpr (kModel)
-                                if (sst) fr0[b] = wz[t][b];
                         }
                     }
                 } else if constexpr (kGrad) {
@@ -604,18 +579,12 @@ __global__ __launch_bounds__(256, (OP == kMttkrp && N == 3 && NB == 1 && G * VEC
                         if (col < R) {
                             float r[VEC], q[VEC], z[VEC];
                             rw[t][b].get(r);
-                            if constexpr (kFactor) {
-                                fr1[b].get(q);
-                                fr0[b].get(z);
-                            } else {
-                                w1[t][b].get(q);
-                                wz[t][b].get(z);
-                            }
+                            w1[t][b].get(q);
+                            wz[t][b].get(z);
 #pragma unroll
                             for (int e = 0; e < VEC; ++e) r[e] *= q[e] * z[e];
                             if constexpr (N == 4) {
-                                if constexpr (kFactor) fr2[b].get(q);
-                                else w2[t][b].get(q);
+                                w2[t][b].get(q);
 #pragma unroll
                                 for (int e = 0; e < VEC; ++e) r[e] *= q[e];
                             }
@@ -779,7 +748,6 @@ __global__ __launch_bounds__(256, (OP == kMttkrp && N == 3 && NB == 1 && G * VEC
             for (int b = 0; b < NB; ++b) {
                 cw1[b] = fr1[b];
                 if constexpr (N == 4) cw2[b] = fr2[b];
-                if constexpr (kModel) cw0[b] = fr0[b];
             }
             i1c = __shfl(ec.i1, 63, 64);
             if constexpr (N == 4) i2c = __shfl(ec.i2, 63, 64);
