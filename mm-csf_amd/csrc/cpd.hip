@@ -436,7 +436,7 @@ static size_t cpd_space(const Format &f, int R, int dtype, char *base, CpdSpace 
     t.sq = (double *)take(R * sizeof(double));
     t.scal = (double *)take(4 * sizeof(double));  // [0] |X|^2, [1] inner, [2] fit
     t.mx = (uint32_t *)take(R * sizeof(uint32_t));
-    t.status = (int *)take(sizeof(int));
+    t.status = (int *)take(kMaxDims * sizeof(int));  // solve status of each mode
     t.M = (float *)take(maxRows * R * sizeof(float));
     for (int m = 0; m < kMaxDims; ++m) t.Ub[m] = nullptr;
     if (dtype == MMCSF_BF16)
@@ -448,8 +448,12 @@ static size_t cpd_space(const Format &f, int R, int dtype, char *base, CpdSpace 
 // The ALS loop on a work space `ws` of cpd_space(...) bytes.  Every part that
 // must start at zero is cleared by a kernel here, so the work space may hold
 // anything on entry.  timeMttkrp: time the MTTKRP launches with HIP events.
+// side (optional, non-blocking): a second stream on which each mode's normal
+// matrix is inverted while that mode's MTTKRP runs (the inverse needs only the
+// other modes' Grams); null runs everything on s.  The host reads the modes'
+// solve status and the fit once per iteration.
 static void cpd_als_on(const Format &f, int R, int iters, double tol, int dtype, float *const *U, double *lambdaHost,
-                       mmcsf_cpd_stats *stats, hipStream_t s, char *ws, bool timeMttkrp) {
+                       mmcsf_cpd_stats *stats, hipStream_t s, char *ws, bool timeMttkrp, hipStream_t side) {
     const int N = f.ndims;
     const size_t RR = (size_t)R * R;
     CpdSpace st{};
@@ -468,7 +472,7 @@ static void cpd_als_on(const Format &f, int R, int iters, double tol, int dtype,
         refresh_bf16(m);
     }
     hipLaunchKernelGGL(k_zero_d, dim3(1), dim3(256), 0, s, st.scal, (uint64_t)4);
-    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, (uint32_t *)st.status, (uint64_t)1);
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, (uint32_t *)st.status, (uint64_t)kMaxDims);
     // |X|^2 over every stored value (the partitions of a mixed MM-CSF)
     if (f.mixed) {
         for (int p = 0; p < N; ++p)
@@ -484,27 +488,47 @@ static void cpd_als_on(const Format &f, int R, int iters, double tol, int dtype,
     F.dtype = dtype;
     F.R = R;
     for (int m = 0; m < N; ++m) F.U[m] = dtype == MMCSF_BF16 ? (const void *)st.Ub[m] : (const void *)U[m];
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    if (timeMttkrp) {
-        MMCSF_HIP(hipEventCreate(&e0));
-        MMCSF_HIP(hipEventCreate(&e1));
-    }
+    // events: MTTKRP start / end per mode (timing), and the two joins of the
+    // side stream (after the previous mode's Gram, after the inverse)
     struct Events {
-        hipEvent_t a, b;
+        hipEvent_t e[2 * kMaxDims + 2] = {};
         ~Events() {
-            if (a) (void)hipEventDestroy(a);
-            if (b) (void)hipEventDestroy(b);
+            for (hipEvent_t x : e)
+                if (x) (void)hipEventDestroy(x);
         }
-    } events{e0, e1};
+    } ev;
+    if (timeMttkrp)
+        for (int i = 0; i < 2 * N; ++i) MMCSF_HIP(hipEventCreate(&ev.e[i]));
+    hipEvent_t evReady = nullptr, evInv = nullptr;
+    if (side) {
+        MMCSF_HIP(hipEventCreateWithFlags(&ev.e[2 * kMaxDims], hipEventDisableTiming));
+        MMCSF_HIP(hipEventCreateWithFlags(&ev.e[2 * kMaxDims + 1], hipEventDisableTiming));
+        evReady = ev.e[2 * kMaxDims];
+        evInv = ev.e[2 * kMaxDims + 1];
+    }
     double mttkrpMs = 0, oldfit = 0, fit = 0;
     int it = 0, lu = 0;
     auto t0 = std::chrono::steady_clock::now();
     for (it = 0; it < iters; ++it) {
         for (int m = 0; m < N; ++m) {
-            mttkrp_mode(f, m, F, st.M, s, e0, e1);
             const size_t ishm = 3 * RR * sizeof(double) <= kInverseLds ? 3 * RR * sizeof(double) : 0;
-            hipLaunchKernelGGL(k_normal_inverse, dim3(1), dim3(256), ishm, s, st.G, N, m, R, st.W, st.Hinv,
-                               st.status);
+            if (side) {
+                // the Grams of the other modes are final once the work queued on
+                // s so far (the previous mode's update) is done; the inverse
+                // then runs beside this mode's MTTKRP
+                MMCSF_HIP(hipEventRecord(evReady, s));
+                MMCSF_HIP(hipStreamWaitEvent(side, evReady, 0));
+                hipLaunchKernelGGL(k_normal_inverse, dim3(1), dim3(256), ishm, side, st.G, N, m, R, st.W, st.Hinv,
+                                   st.status + m);
+                MMCSF_HIP(hipEventRecord(evInv, side));
+            }
+            mttkrp_mode(f, m, F, st.M, s, timeMttkrp ? ev.e[2 * m] : nullptr, timeMttkrp ? ev.e[2 * m + 1] : nullptr);
+            if (side) {
+                MMCSF_HIP(hipStreamWaitEvent(s, evInv, 0));
+            } else {
+                hipLaunchKernelGGL(k_normal_inverse, dim3(1), dim3(256), ishm, s, st.G, N, m, R, st.W, st.Hinv,
+                                   st.status + m);
+            }
             const size_t shm = RR * sizeof(double) <= 32768 ? RR * sizeof(double) : 0;
             hipLaunchKernelGGL(k_apply_inverse, dim3(grid_for(f.dims[m] * ((R + 3) / 4), 8192)), dim3(256), shm, s, st.M,
                                st.Hinv, (uint32_t)f.dims[m], R, U[m]);
@@ -524,20 +548,24 @@ static void cpd_als_on(const Format &f, int R, int iters, double tol, int dtype,
                                    (uint32_t)f.dims[m], R, st.lam, st.scal + 1);
             }
             MMCSF_HIP(hipGetLastError());
-            int stat = 0;
-            MMCSF_HIP(hipMemcpyAsync(&stat, st.status, sizeof(int), hipMemcpyDeviceToHost, s));
-            MMCSF_HIP(hipStreamSynchronize(s));
-            if (timeMttkrp) {
-                float ms = 0;
-                MMCSF_HIP(hipEventElapsedTime(&ms, e0, e1));
-                mttkrpMs += ms;
-            }
-            if (stat == 1) ++lu;
-            MMCSF_REQUIRE(stat != 2, MMCSF_ERR_UNSUPPORTED, "normal equations are singular");
         }
         hipLaunchKernelGGL(k_fit, dim3(1), dim3(256), 0, s, st.G, N, R, st.lam, st.scal + 1, st.scal, st.scal + 2);
+        int stat[kMaxDims] = {0, 0, 0, 0};
         MMCSF_HIP(hipMemcpyAsync(&fit, st.scal + 2, sizeof(double), hipMemcpyDeviceToHost, s));
+        MMCSF_HIP(hipMemcpyAsync(stat, st.status, N * sizeof(int), hipMemcpyDeviceToHost, s));
         MMCSF_HIP(hipStreamSynchronize(s));
+        // a singular solve leaves that mode's factor (and the later modes')
+        // undefined: the call fails
+        for (int m = 0; m < N; ++m) {
+            if (stat[m] == 1) ++lu;
+            MMCSF_REQUIRE(stat[m] != 2, MMCSF_ERR_UNSUPPORTED, "normal equations are singular");
+        }
+        if (timeMttkrp)
+            for (int m = 0; m < N; ++m) {
+                float ms = 0;
+                MMCSF_HIP(hipEventElapsedTime(&ms, ev.e[2 * m], ev.e[2 * m + 1]));
+                mttkrpMs += ms;
+            }
         if (stats && it < (int)(sizeof(stats->fit_history) / sizeof(double))) stats->fit_history[it] = fit;
         if (it > 0 && std::fabs(fit - oldfit) < tol) {  // cpd_cpu.cu:670-672
             ++it;
@@ -572,7 +600,15 @@ static void cpd_als_on(const Format &f, int R, int iters, double tol, int dtype,
 void cpd_als(const Format &f, int R, int iters, double tol, int dtype, float *const *U, double *lambdaHost,
              mmcsf_cpd_stats *stats, hipStream_t s) {
     DevArray<char> ws(cpd_space(f, R, dtype, nullptr, nullptr));
-    cpd_als_on(f, R, iters, tol, dtype, U, lambdaHost, stats, s, ws.p, true);
+    // the stream the normal-matrix inverses run on (default priority)
+    struct Side {
+        hipStream_t s = nullptr;
+        ~Side() {
+            if (s) (void)hipStreamDestroy(s);
+        }
+    } side;
+    MMCSF_HIP(hipStreamCreateWithFlags(&side.s, hipStreamNonBlocking));
+    cpd_als_on(f, R, iters, tol, dtype, U, lambdaHost, stats, s, ws.p, true, side.s);
 }
 
 // ---------------------------------------------------------------------------
@@ -615,7 +651,7 @@ void cpd_als_noalloc(const Format &f, int R, int iters, double tol, int dtype, f
                       "-byte scratch of the allocation-free entry point (largest mode rows x R x 4 bytes, plus "
                       "the Grams and, for bfloat16 MTTKRP, the bfloat16 factor copies)");
     char *ws = scratch_on_current_device();
-    cpd_als_on(f, R, iters, tol, dtype, U, lambdaHost, stats, s, ws, false);
+    cpd_als_on(f, R, iters, tol, dtype, U, lambdaHost, stats, s, ws, false, nullptr);
 }
 
 }  // namespace mmcsf
