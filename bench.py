@@ -758,7 +758,7 @@ def run_c5(args):
         init = [0.3 * torch.randn((d, R), generator=g2, device=dev) for d in dims]
         for bf16 in (False, True):
             P = [u.clone().requires_grad_(True) for u in init]
-            opt = torch.optim.Adam(P, lr=1e-2)
+            opt = torch.optim.Adam(P, lr=1e-2, fused=True)  # one pass per parameter tensor
             losses = []
 
             def step():
