@@ -215,6 +215,49 @@ __global__ void k_normal_inverse(const double *__restrict__ G, int N, int skip, 
 
 // U[i][j] = sum_k M[i][k] Hinv[k][j] (float64 accumulation); each thread
 // computes four columns of one row, reading M[i][k] once for all four
+// The same product for R <= 32 with R % 4 == 0: one row per thread (the grid
+// covers the rows once, so nothing is loop-invariant across rows and the
+// Hinv reads stay in the k loop), read with eight 16-byte loads; Hinv
+// zero-padded to 32 x 32 in LDS (every lane of a wave reads the same entry),
+// 32 float64 sums per thread.
+__global__ __launch_bounds__(256) void k_apply_inverse32(const float *__restrict__ M, const double *__restrict__ Hinv,
+                                                         uint32_t rows, int R, float *__restrict__ U) {
+    __shared__ double hs[32 * 32];
+    for (int e = threadIdx.x; e < 32 * 32; e += blockDim.x) {
+        const int k = e / 32, j = e % 32;
+        hs[e] = (k < R && j < R) ? Hinv[k * R + j] : 0.0;
+    }
+    __syncthreads();
+    const uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (i >= rows) return;
+    const int q4 = R / 4;  // float4 per row
+    const float4 *m4 = reinterpret_cast<const float4 *>(M + i * R);
+    float row[32];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const float4 v = q < q4 ? m4[q] : make_float4(0.f, 0.f, 0.f, 0.f);
+        row[4 * q] = v.x; row[4 * q + 1] = v.y; row[4 * q + 2] = v.z; row[4 * q + 3] = v.w;
+    }
+    double s[32];
+#pragma unroll
+    for (int j = 0; j < 32; ++j) s[j] = 0.0;
+#pragma unroll 2
+    for (int k = 0; k < 32; ++k) {
+        const double mk = (double)row[k];
+        const double2 *h2 = reinterpret_cast<const double2 *>(hs + k * 32);
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            const double2 h = h2[j];
+            s[2 * j] = fma(mk, h.x, s[2 * j]);
+            s[2 * j + 1] = fma(mk, h.y, s[2 * j + 1]);
+        }
+    }
+    float4 *u4 = reinterpret_cast<float4 *>(U + i * R);
+#pragma unroll
+    for (int q = 0; q < 8; ++q)
+        if (q < q4) u4[q] = make_float4((float)s[4 * q], (float)s[4 * q + 1], (float)s[4 * q + 2], (float)s[4 * q + 3]);
+}
+
 __global__ __launch_bounds__(256) void k_apply_inverse(const float *__restrict__ M, const double *__restrict__ Hinv,
                                                        uint32_t rows, int R, float *__restrict__ U) {
     extern __shared__ double hs[];
@@ -529,9 +572,14 @@ static void cpd_als_on(const Format &f, int R, int iters, double tol, int dtype,
                 hipLaunchKernelGGL(k_normal_inverse, dim3(1), dim3(256), ishm, s, st.G, N, m, R, st.W, st.Hinv,
                                    st.status + m);
             }
-            const size_t shm = RR * sizeof(double) <= 32768 ? RR * sizeof(double) : 0;
-            hipLaunchKernelGGL(k_apply_inverse, dim3(grid_for(f.dims[m] * ((R + 3) / 4), 8192)), dim3(256), shm, s, st.M,
-                               st.Hinv, (uint32_t)f.dims[m], R, U[m]);
+            if (R <= 32 && R % 4 == 0 && reinterpret_cast<uintptr_t>(U[m]) % 16 == 0) {  // 16-byte row stores
+                hipLaunchKernelGGL(k_apply_inverse32, dim3((unsigned)((f.dims[m] + 255) / 256)), dim3(256), 0, s, st.M,
+                                   st.Hinv, (uint32_t)f.dims[m], R, U[m]);
+            } else {
+                const size_t shm = RR * sizeof(double) <= 32768 ? RR * sizeof(double) : 0;
+                hipLaunchKernelGGL(k_apply_inverse, dim3(grid_for(f.dims[m] * ((R + 3) / 4), 8192)), dim3(256), shm, s,
+                                   st.M, st.Hinv, (uint32_t)f.dims[m], R, U[m]);
+            }
             const int kind = it == 0 ? 0 : 1;  // 2-norm at iteration 0, max-norm after
             clear_norms();
             hipLaunchKernelGGL(k_colnorm, dim3(colnorm_grid(f.dims[m], R)), dim3(256), 0, s, U[m],
