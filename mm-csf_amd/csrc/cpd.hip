@@ -350,9 +350,15 @@ __global__ __launch_bounds__(256) void k_dot_lambda(const float *__restrict__ A,
                                                     double *__restrict__ acc) {
     double s = 0.0;
     const uint64_t total = (uint64_t)rows * R;
-    for (uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < total;
-         e += (uint64_t)gridDim.x * blockDim.x)
-        s += (double)A[e] * (double)B[e] * lam[e % R];
+    if (total < 0x80000000ull) {  // 32-bit indices and column modulo (e + step stays below 2^32)
+        const uint32_t n = (uint32_t)total, r = (uint32_t)R, step = gridDim.x * blockDim.x;
+        for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += step)
+            s += (double)A[e] * (double)B[e] * lam[e % r];
+    } else {
+        for (uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < total;
+             e += (uint64_t)gridDim.x * blockDim.x)
+            s += (double)A[e] * (double)B[e] * lam[e % R];
+    }
     __shared__ double red[256];
     red[threadIdx.x] = s;
     __syncthreads();
@@ -592,7 +598,7 @@ static void cpd_als_on(const Format &f, int R, int iters, double tol, int dtype,
             if (m == N - 1) {
                 // inner product of the last MTTKRP with the updated last factor
                 hipLaunchKernelGGL(k_zero_d, dim3(1), dim3(64), 0, s, st.scal + 1, (uint64_t)1);
-                hipLaunchKernelGGL(k_dot_lambda, dim3(grid_for(f.dims[m] * R, 4096)), dim3(256), 0, s, U[m], st.M,
+                hipLaunchKernelGGL(k_dot_lambda, dim3(grid_for(f.dims[m] * R, 512)), dim3(256), 0, s, U[m], st.M,
                                    (uint32_t)f.dims[m], R, st.lam, st.scal + 1);
             }
             MMCSF_HIP(hipGetLastError());
