@@ -442,7 +442,7 @@ uint64_t csf_bytes(const mmcsf::Csf &c) {
 }
 
 uint64_t format_bytes(const mmcsf::Format &f) {
-    uint64_t b = csf_bytes(f.csf) + bytes_of(f.cooVals) + csf_bytes(f.hyb.csf) + bytes_of(f.hyb.cooVal) +
+    uint64_t b = csf_bytes(f.csf) + bytes_of(f.cooVals) + csf_bytes(f.hyb.csf) + csf_bytes(f.hyb.cslWalk) + bytes_of(f.hyb.cooVal) +
                  bytes_of(f.hyb.cslPtr) + bytes_of(f.hyb.cslIdx) + bytes_of(f.hyb.cslInd1) +
                  bytes_of(f.hyb.cslInd2) + bytes_of(f.hyb.cslVal) + bytes_of(f.hyb.cslBinList);
     for (int l = 0; l < mmcsf::kMaxDims; ++l) {
@@ -861,6 +861,7 @@ int mmcsf_format_build(const mmcsf_tensor *th, int type, const mmcsf_options *op
                         for (int l = 1; l < t.ndims; ++l) build_hot_table(f.csf, l, t.dims[order[l]], f.csf.hot[l], s);
                 } else {
                     build_hyb(t.ndims, t.dims, std::move(f.csf), f.hyb, s);
+                    build_csl_walk(f.hyb, s);
                 }
             }
         }

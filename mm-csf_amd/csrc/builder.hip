@@ -740,6 +740,52 @@ __global__ void k_compose(const uint32_t *__restrict__ inner, const uint32_t *__
 
 }  // namespace
 
+// level-1 entry x of the CSL walk tree (one per nonzero) takes its slice's
+// coordinate
+__global__ void k_csl_slice_of(const uint32_t *__restrict__ ptr, const uint32_t *__restrict__ idx, uint64_t nS,
+                               uint32_t *slc) {
+    uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (i >= nS) return;
+    for (uint32_t x = ptr[i]; x < ptr[i + 1]; ++x) slc[x] = idx[i];
+}
+
+// The CSL part (slices whose leaf fibers all hold one nonzero) is a CSF tree
+// with one level-1 entry per nonzero: ptr[0] = cslPtr, idx[0] = cslIdx,
+// ptr[1] = 0, 1, ..., idx[1] = cslInd1, leaf = cslInd2.  Walked by the chunked
+// root walk like the CSF part; the CSL arrays themselves stay as built.
+void build_csl_walk(Hyb &h, hipStream_t s) {
+    Csf &t = h.cslWalk;
+    t = Csf{};
+    if (h.ndims != 3 || h.cslNnz == 0) return;
+    const uint64_t n = h.cslNnz, nS = h.nCsl;
+    t.ndims = 3;
+    for (int l = 0; l < 3; ++l) {
+        t.order[l] = h.order[l];
+        t.rows[l] = h.csf.rows[l];
+    }
+    t.nnz = n;
+    t.cnt[0] = nS;
+    t.cnt[1] = n;
+    t.cnt[2] = n;
+    auto copy = [&](DevArray<uint32_t> &dst, const uint32_t *src, uint64_t count) {
+        dst.alloc(count);
+        MMCSF_HIP(hipMemcpyAsync(dst.p, src, count * 4, hipMemcpyDeviceToDevice, s));
+    };
+    copy(t.ptr[0], h.cslPtr.p, nS + 1);
+    copy(t.idx[0], h.cslIdx.p, nS);
+    t.ptr[1].alloc(n + 1);
+    hipLaunchKernelGGL(k_iota, dim3(nblk(n + 1)), dim3(kBS), 0, s, t.ptr[1].p, n + 1);
+    copy(t.idx[1], h.cslInd1.p, n);
+    copy(t.leaf, h.cslInd2.p, n);
+    t.vals.alloc(n);
+    MMCSF_HIP(hipMemcpyAsync(t.vals.p, h.cslVal.p, n * 4, hipMemcpyDeviceToDevice, s));
+    t.fbrSlc.alloc(n);
+    hipLaunchKernelGGL(k_csl_slice_of, dim3(nblk(nS)), dim3(kBS), 0, s, h.cslPtr.p, h.cslIdx.p, nS, t.fbrSlc.p);
+    MMCSF_HIP(hipGetLastError());
+    MMCSF_HIP(hipStreamSynchronize(s));
+    build_walk_bits(t, s);
+}
+
 void build_hyb(int N, const uint64_t *dims, Csf &&src, Hyb &h, hipStream_t s) {
     (void)dims;
     Temp tmp;

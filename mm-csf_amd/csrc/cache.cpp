@@ -297,6 +297,8 @@ void format_load(const char *path, Format &f) {
     MMCSF_REQUIRE(r.pod<uint32_t>() == kByteOrder, MMCSF_ERR_IO, "format cache written with another byte order");
     format_fields(r, f);
     st.drain();
+    // derived from the stored CSL arrays: rebuilt, not stored
+    build_csl_walk(f.hyb, st.s);
     r.bytes(m, 8);
     MMCSF_REQUIRE(std::memcmp(m, kEnd, 8) == 0 && r.left == 0, MMCSF_ERR_IO, "format cache: bad end of file");
     MMCSF_REQUIRE(f.ndims == 3 || f.ndims == 4, MMCSF_ERR_IO, "format cache: bad tensor order");

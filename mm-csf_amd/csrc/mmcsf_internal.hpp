@@ -136,6 +136,9 @@ struct Hyb {
     DevArray<uint32_t> cslBinList;
     uint64_t cslBinCount[kNumBins] = {0};
     Csf csf;                               // CSF part (hasBins = true)
+    // 3-D: the CSL part as a tree for the chunked walk (derived from the CSL
+    // arrays: slices, one level-1 entry per nonzero); empty otherwise
+    Csf cslWalk;
 };
 
 struct Format {
@@ -187,6 +190,8 @@ void build_csf(int ndims, const uint64_t *dims, uint64_t nnz, const int *order,
 
 void build_bins_tiled(Csf &c, hipStream_t s);           // make_TiledBin
 void build_hyb(int ndims, const uint64_t *dims, Csf &&src, Hyb &out, hipStream_t s);
+// the CSL part of a built 3-D HB-CSF as a walkable tree (h.cslWalk)
+void build_csl_walk(Hyb &h, hipStream_t s);
 void build_hot_table(const Csf &c, int level, uint64_t rows, HotTable &h, hipStream_t s);
 // rows of tensor mode m holding `frac` of the nonzeros, most used first
 uint64_t hot_footprint(const Tensor &t, int m, double frac, hipStream_t s);

@@ -624,6 +624,7 @@ void mttkrp_hyb(const Hyb &h, const uint64_t *dims, const Factors &F, float *out
     // the CSF part's walk loads wider rows than the COO / CSL kernels: refuse
     // its misaligned factors before any part is launched
     if (h.csf.nnz && walk_supports(h.csf, F.R)) check_walk_align(h.csf, F, 0, out);
+    if (h.cslWalk.nnz && walk_supports(h.cslWalk, F.R)) check_walk_align(h.cslWalk, F, 0, out);
     fill_zero(out, dims[h.order[0]] * (uint64_t)F.R, s);
     // COO part: slices with one nonzero; each output row is written once
     if (h.cooNnz) {
@@ -647,8 +648,11 @@ void mttkrp_hyb(const Hyb &h, const uint64_t *dims, const Factors &F, float *out
             else run_geo<CooSel<4>::template K, CooArgs, uint16_t>(g, a, groups, s);
         }
     }
-    // CSL part (3-D)
-    if (h.cslNnz) {
+    // CSL part (3-D): walked as a tree of one-nonzero fibers; the CSL kernel
+    // where the walk cannot run (R % 4 != 0, tables above 4 GiB)
+    if (h.cslNnz && h.cslWalk.nnz && walk_supports(h.cslWalk, F.R)) {
+        mttkrp_root(h.cslWalk, F, out, s);
+    } else if (h.cslNnz) {
         CslArgs a{};
         a.ptr = h.cslPtr.p;
         a.idx = h.cslIdx.p;
