@@ -105,10 +105,11 @@ __global__ void k_zero_d(double *p, uint64_t n) {
 // when the Cholesky factorization fails (MatrixSolveNormals,
 // cpd_cpu.cu:140-208).  One workgroup; W is an R*R*3 float64 workspace, and
 // for R <= 36 the three matrices live in LDS instead.  Cholesky (lower), the
-// inverse X of L, then H^-1 = X^T X.
+// inverse X of L, then H^-1 = X^T X.  Hpad (optional, R <= 32) receives H^-1
+// zero-padded to 32 x 32.
 // ---------------------------------------------------------------------------
 __global__ void k_normal_inverse(const double *__restrict__ G, int N, int skip, int R, double *__restrict__ W,
-                                 double *__restrict__ Hinv, int *__restrict__ status) {
+                                 double *__restrict__ Hinv, double *__restrict__ Hpad, int *__restrict__ status) {
     const int tid = threadIdx.x, nt = blockDim.x;
     // H, L and X live in LDS when the launch provides 3 R*R doubles of it
     // (every elimination step then waits on LDS, not on L2), else in W
@@ -172,7 +173,11 @@ __global__ void k_normal_inverse(const double *__restrict__ G, int N, int skip, 
             double s = 0.0;
             for (int k = max(i, j); k < R; ++k) s += X[(size_t)k * R + i] * X[(size_t)k * R + j];
             Hinv[e] = s;
+            if (Hpad) Hpad[i * 32 + j] = s;
         }
+        if (Hpad)
+            for (int e = tid; e < 32 * 32; e += nt)
+                if (e / 32 >= R || e % 32 >= R) Hpad[e] = 0.0;
         if (tid == 0) status[0] = 0;
         return;
     }
@@ -210,6 +215,9 @@ __global__ void k_normal_inverse(const double *__restrict__ G, int N, int skip, 
                     for (int k = i + 1; k < R; ++k) s -= L[(size_t)i * R + k] * Hinv[(size_t)k * R + c];
                     Hinv[(size_t)i * R + c] = s / L[(size_t)i * R + i];
                 }
+        if (Hpad)
+            for (int e = 0; e < 32 * 32; ++e)
+                Hpad[e] = (e / 32 < R && e % 32 < R) ? Hinv[(e / 32) * R + e % 32] : 0.0;
     }
 }
 
@@ -220,42 +228,206 @@ __global__ void k_normal_inverse(const double *__restrict__ G, int N, int skip, 
 // Hinv reads stay in the k loop), read with eight 16-byte loads; Hinv
 // zero-padded to 32 x 32 in LDS (every lane of a wave reads the same entry),
 // 32 float64 sums per thread.
-__global__ __launch_bounds__(256) void k_apply_inverse32(const float *__restrict__ M, const double *__restrict__ Hinv,
-                                                         uint32_t rows, int R, float *__restrict__ U) {
-    __shared__ double hs[32 * 32];
-    for (int e = threadIdx.x; e < 32 * 32; e += blockDim.x) {
-        const int k = e / 32, j = e % 32;
-        hs[e] = (k < R && j < R) ? Hinv[k * R + j] : 0.0;
-    }
-    __syncthreads();
-    const uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
-    if (i >= rows) return;
+// U = M H^-1 for R <= 32 (R % 4 == 0), one row per thread, fused with the
+// column norms of the new rows: kind 0 adds float64 sums of squares to sq,
+// kind 1 the largest non-negative entry (order-preserving bits) to mx.  The
+// block's rows are one contiguous run of M and U: they are read and written
+// as consecutive float4 and pass through LDS (33 floats per row), where each
+// thread solves its row and the block reduces the columns, so that each block
+// adds one value per column.  H^-1 (Hpad: zero-padded to 32 x 32) is the same
+// for every lane: it is read with scalar loads and enters the float64 FMAs as
+// scalar operands.  Block 0 also clears Gz, the R x R Gram that
+// k_scale_gram32 accumulates next.
+constexpr int kSolveThreads = 512;
+__global__ __launch_bounds__(kSolveThreads) void k_solve_rows32(const float *__restrict__ M,
+                                                                const double *__restrict__ Hpad, uint32_t rows,
+                                                                int R, float *__restrict__ U, int kind,
+                                                                double *__restrict__ sq, uint32_t *__restrict__ mx,
+                                                                double *__restrict__ Gz) {
+    __shared__ float tile[kSolveThreads * 33];
+    __shared__ double ps[kSolveThreads];
+    using ConstD = const __attribute__((address_space(4))) double;
+    const ConstD *H = reinterpret_cast<const ConstD *>(reinterpret_cast<uintptr_t>(Hpad));
+    if (blockIdx.x == 0)
+        for (int e = threadIdx.x; e < R * R; e += blockDim.x) Gz[e] = 0.0;
+    const uint64_t row0 = (uint64_t)blockIdx.x * kSolveThreads;
+    const uint32_t nrows = (uint32_t)min((uint64_t)kSolveThreads, rows - row0);
     const int q4 = R / 4;  // float4 per row
-    const float4 *m4 = reinterpret_cast<const float4 *>(M + i * R);
-    float row[32];
+    const uint32_t n4 = nrows * (uint32_t)q4;
+    // at most 8 float4 per thread (n4 <= 512 rows x 8): every read is issued
+    // before the first is waited for
+    constexpr int kPer = 8;
+    const float4 *m4 = reinterpret_cast<const float4 *>(M + row0 * R);
+    float4 v[kPer];
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
-        const float4 v = q < q4 ? m4[q] : make_float4(0.f, 0.f, 0.f, 0.f);
-        row[4 * q] = v.x; row[4 * q + 1] = v.y; row[4 * q + 2] = v.z; row[4 * q + 3] = v.w;
+    for (int q = 0; q < kPer; ++q) {
+        const uint32_t e = threadIdx.x + q * kSolveThreads;
+        v[q] = e < n4 ? m4[e] : make_float4(0.f, 0.f, 0.f, 0.f);
     }
-    double s[32];
 #pragma unroll
-    for (int j = 0; j < 32; ++j) s[j] = 0.0;
-#pragma unroll 2
-    for (int k = 0; k < 32; ++k) {
-        const double mk = (double)row[k];
-        const double2 *h2 = reinterpret_cast<const double2 *>(hs + k * 32);
-#pragma unroll
-        for (int j = 0; j < 16; ++j) {
-            const double2 h = h2[j];
-            s[2 * j] = fma(mk, h.x, s[2 * j]);
-            s[2 * j + 1] = fma(mk, h.y, s[2 * j + 1]);
+    for (int q = 0; q < kPer; ++q) {
+        const uint32_t e = threadIdx.x + q * kSolveThreads;
+        if (e < n4) {
+            float *t = tile + (e / q4) * 33 + (e % q4) * 4;
+            t[0] = v[q].x; t[1] = v[q].y; t[2] = v[q].z; t[3] = v[q].w;
         }
     }
-    float4 *u4 = reinterpret_cast<float4 *>(U + i * R);
+    __syncthreads();
+    const bool live = threadIdx.x < nrows;
+    float *mine = tile + threadIdx.x * 33;
+    // the row times H^-1, 16 columns at a time (16 float64 sums live)
+    float out[32];
 #pragma unroll
-    for (int q = 0; q < 8; ++q)
-        if (q < q4) u4[q] = make_float4((float)s[4 * q], (float)s[4 * q + 1], (float)s[4 * q + 2], (float)s[4 * q + 3]);
+    for (int half = 0; half < 2; ++half) {
+        double s[16];
+#pragma unroll
+        for (int j = 0; j < 16; ++j) s[j] = 0.0;
+#pragma unroll 4
+        for (int k = 0; k < 32; ++k) {
+            const double mk = k < R ? (double)mine[k] : 0.0;
+#pragma unroll
+            for (int j = 0; j < 16; ++j) s[j] = fma(mk, H[k * 32 + half * 16 + j], s[j]);
+        }
+#pragma unroll
+        for (int j = 0; j < 16; ++j) out[half * 16 + j] = (float)s[j];
+    }
+    // the new row replaces the thread's own row (rows past the factor and
+    // columns past R hold zero)
+#pragma unroll
+    for (int j = 0; j < 32; ++j) mine[j] = (live && j < R) ? out[j] : 0.f;
+    __syncthreads();
+    float4 *u4 = reinterpret_cast<float4 *>(U + row0 * R);
+#pragma unroll
+    for (int q = 0; q < kPer; ++q) {
+        const uint32_t e = threadIdx.x + q * kSolveThreads;
+        if (e < n4) {
+            const float *t = tile + (e / q4) * 33 + (e % q4) * 4;
+            u4[e] = make_float4(t[0], t[1], t[2], t[3]);
+        }
+    }
+    // thread t: column t % 32 over rows t / 32, t / 32 + 16, ...
+    constexpr int kSubs = kSolveThreads / 32;
+    const int j = threadIdx.x % 32, sub = threadIdx.x / 32;
+    double ss = 0.0;
+    float mm = 0.f;
+    for (int r = sub; r < kSolveThreads; r += kSubs) {
+        const float v = tile[r * 33 + j];
+        ss += (double)v * v;
+        mm = fmaxf(mm, v);
+    }
+    __syncthreads();  // rows no longer read: the maxima reuse the front of the tile
+    float *pm = tile;
+    ps[sub * 32 + j] = ss;
+    pm[sub * 32 + j] = mm;
+    __syncthreads();
+    if (threadIdx.x < R) {  // sub 0, column j
+        for (int q = 1; q < kSubs; ++q) {
+            ss += ps[q * 32 + j];
+            mm = fmaxf(mm, pm[q * 32 + j]);
+        }
+        if (kind == 0) unsafeAtomicAdd(sq + j, ss);
+        else atomicMax(mx + j, __float_as_uint(mm));
+    }
+}
+
+// lambda of column c from the reduced norms: the 2-norm (kind 0) or the
+// largest entry clamped to at least 1 (kind 1, MatrixMaxNorm, cpd_cpu.cu:316-319)
+__device__ inline double lambda_of(int kind, const double *sq, const uint32_t *mx, int c) {
+    return kind == 0 ? sqrt(sq[c]) : fmax((double)__uint_as_float(mx[c]), 1.0);
+}
+
+__device__ inline uint16_t to_bf16(float x) {
+    uint32_t u = __float_as_uint(x);
+    u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even
+    return (uint16_t)(u >> 16);
+}
+
+// The normalisation and Gram of a factor (R <= 32): lambda from the column
+// norms, U[:, c] /= lambda_c in place, Ub (optional) = its bfloat16 copy, and
+// G += U^T U of the normalised rows.  Every wave owns kScaleRows rows and
+// all their columns (the 16 x 16 tiles cols 0-15 x 0-15, 0-15 x 16-31 and
+// 16-31 x 16-31, MFMA operands laid out as in k_gram), so each element is
+// read, scaled and written by one lane.  The block's 16 waves are summed in
+// LDS before the block adds its tiles to G.  Block 0 writes lambda and clears
+// sqNext / mxNext, the norm accumulators of the next update.
+constexpr int kScaleThreads = 1024;
+constexpr int kScaleRows = 64;
+__global__ __launch_bounds__(kScaleThreads) void k_scale_gram32(float *__restrict__ U, uint32_t rows, int R,
+                                                      int kind, const double *__restrict__ sq,
+                                                      const uint32_t *__restrict__ mx, double *__restrict__ lam,
+                                                      double *__restrict__ sqNext, uint32_t *__restrict__ mxNext,
+                                                      uint16_t *__restrict__ Ub, double *__restrict__ G) {
+    constexpr int kWaves = kScaleThreads / 64;
+    __shared__ float part[3][kWaves][256];
+    if (blockIdx.x == 0 && threadIdx.x < R) {
+        lam[threadIdx.x] = lambda_of(kind, sq, mx, threadIdx.x);
+        sqNext[threadIdx.x] = 0.0;
+        mxNext[threadIdx.x] = 0u;
+    }
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const bool two = R > 16;
+    const int c0 = lane & 15, c1 = 16 + (lane & 15);
+    const bool ok0 = c0 < R, ok1 = c1 < R;
+    double d0 = ok0 ? lambda_of(kind, sq, mx, c0) : 1.0, d1 = ok1 ? lambda_of(kind, sq, mx, c1) : 1.0;
+    if (d0 == 0.0) d0 = 1.0;
+    if (d1 == 0.0) d1 = 1.0;
+    const uint64_t wave = (uint64_t)blockIdx.x * kWaves + wid;
+    const uint64_t r0 = wave * kScaleRows;
+    const uint64_t r1 = min((uint64_t)rows, r0 + kScaleRows);
+    const int kk = lane >> 4;
+    v4f a00 = {0.f, 0.f, 0.f, 0.f}, a01 = a00, a11 = a00;
+    for (uint64_t k = r0; k < r1; k += 16) {
+        float x0[4], x1[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const uint64_t row = k + 4 * q + kk;
+            const bool in = row < r1;
+            x0[q] = in && ok0 ? U[row * R + c0] : 0.f;
+            x1[q] = in && ok1 ? U[row * R + c1] : 0.f;
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const uint64_t row = k + 4 * q + kk;
+            const bool in = row < r1;
+            x0[q] = (float)((double)x0[q] / d0);
+            x1[q] = (float)((double)x1[q] / d1);
+            if (in && ok0) {
+                U[row * R + c0] = x0[q];
+                if (Ub) Ub[row * R + c0] = to_bf16(x0[q]);
+            }
+            if (in && ok1) {
+                U[row * R + c1] = x1[q];
+                if (Ub) Ub[row * R + c1] = to_bf16(x1[q]);
+            }
+            a00 = __builtin_amdgcn_mfma_f32_16x16x4f32(x0[q], x0[q], a00, 0, 0, 0);
+            if (two) {
+                a01 = __builtin_amdgcn_mfma_f32_16x16x4f32(x0[q], x1[q], a01, 0, 0, 0);
+                a11 = __builtin_amdgcn_mfma_f32_16x16x4f32(x1[q], x1[q], a11, 0, 0, 0);
+            }
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        part[0][wid][lane * 4 + r] = a00[r];
+        part[1][wid][lane * 4 + r] = a01[r];
+        part[2][wid][lane * 4 + r] = a11[r];
+    }
+    __syncthreads();
+    // tile t of the wave: t = 0 (0,0), 1 (0,1), 2 (1,1)
+    if (wid < (two ? 3 : 1)) {
+        const int bi = wid == 2 ? 1 : 0, bj = wid == 0 ? 0 : 1;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            float v = 0.f;
+            for (int w = 0; w < kWaves; ++w) v += part[wid][w][lane * 4 + r];
+            const int i = 16 * bi + 4 * (lane >> 4) + r;
+            const int j = 16 * bj + (lane & 15);
+            if (i < R && j < R) {
+                unsafeAtomicAdd(G + (size_t)i * R + j, (double)v);
+                if (bi != bj) unsafeAtomicAdd(G + (size_t)j * R + i, (double)v);
+            }
+        }
+    }
 }
 
 __global__ __launch_bounds__(256) void k_apply_inverse(const float *__restrict__ M, const double *__restrict__ Hinv,
@@ -455,7 +627,7 @@ __global__ void k_zero_u32(uint32_t *p, uint64_t n) {
 // updated, and (bfloat16 MTTKRP) the bfloat16 factor copies; every part
 // 256-byte aligned.
 struct CpdSpace {
-    double *G, *W, *Hinv, *lam, *lamFinal, *sq, *scal;
+    double *G, *W, *Hinv, *Hpad, *lam, *lamFinal, *sq, *scal;
     uint32_t *mx;
     int *status;
     float *M;
@@ -480,11 +652,12 @@ static size_t cpd_space(const Format &f, int R, int dtype, char *base, CpdSpace 
     t.G = (double *)take(N * RR * sizeof(double));
     t.W = (double *)take(3 * RR * sizeof(double));
     t.Hinv = (double *)take(RR * sizeof(double));
+    t.Hpad = (double *)take(32 * 32 * sizeof(double));
     t.lam = (double *)take(R * sizeof(double));
     t.lamFinal = (double *)take(R * sizeof(double));
-    t.sq = (double *)take(R * sizeof(double));
+    t.sq = (double *)take(2 * R * sizeof(double));  // two halves: this update's and the next one's
     t.scal = (double *)take(4 * sizeof(double));  // [0] |X|^2, [1] inner, [2] fit
-    t.mx = (uint32_t *)take(R * sizeof(uint32_t));
+    t.mx = (uint32_t *)take(2 * R * sizeof(uint32_t));
     t.status = (int *)take(kMaxDims * sizeof(int));  // solve status of each mode
     t.M = (float *)take(maxRows * R * sizeof(float));
     for (int m = 0; m < kMaxDims; ++m) t.Ub[m] = nullptr;
@@ -512,9 +685,11 @@ static void cpd_als_on(const Format &f, int R, int iters, double tol, int dtype,
             hipLaunchKernelGGL(k_f32_to_bf16, dim3(grid_for(f.dims[m] * R, 1u << 30)), dim3(256), 0, s, U[m],
                                st.Ub[m], (uint64_t)f.dims[m] * R);
     };
+    // both halves of the norm accumulators; update u of the loop accumulates
+    // into half u % 2, which is zero when it starts
     auto clear_norms = [&]() {
-        hipLaunchKernelGGL(k_zero_d, dim3(1), dim3(256), 0, s, st.sq, (uint64_t)R);
-        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(256), 0, s, st.mx, (uint64_t)R);
+        hipLaunchKernelGGL(k_zero_d, dim3(1), dim3(256), 0, s, st.sq, (uint64_t)2 * R);
+        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(256), 0, s, st.mx, (uint64_t)2 * R);
     };
     for (int m = 0; m < N; ++m) {
         gram(U[m], f.dims[m], R, st.G + m * RR, s);
@@ -557,10 +732,15 @@ static void cpd_als_on(const Format &f, int R, int iters, double tol, int dtype,
     }
     double mttkrpMs = 0, oldfit = 0, fit = 0;
     int it = 0, lu = 0;
+    uint64_t upd = 0;  // factor updates so far (selects the half of the norm accumulators)
+    clear_norms();
     auto t0 = std::chrono::steady_clock::now();
     for (it = 0; it < iters; ++it) {
         for (int m = 0; m < N; ++m) {
             const size_t ishm = 3 * RR * sizeof(double) <= kInverseLds ? 3 * RR * sizeof(double) : 0;
+            // the fused row solve (R <= 32, 16-byte rows) reads H^-1 zero-padded
+            const bool fused = R <= 32 && R % 4 == 0 && reinterpret_cast<uintptr_t>(U[m]) % 16 == 0;
+            double *hpad = fused ? st.Hpad : nullptr;
             if (side) {
                 // the Grams of the other modes are final once the work queued on
                 // s so far (the previous mode's update) is done; the inverse
@@ -568,7 +748,7 @@ static void cpd_als_on(const Format &f, int R, int iters, double tol, int dtype,
                 MMCSF_HIP(hipEventRecord(evReady, s));
                 MMCSF_HIP(hipStreamWaitEvent(side, evReady, 0));
                 hipLaunchKernelGGL(k_normal_inverse, dim3(1), dim3(256), ishm, side, st.G, N, m, R, st.W, st.Hinv,
-                                   st.status + m);
+                                   hpad, st.status + m);
                 MMCSF_HIP(hipEventRecord(evInv, side));
             }
             mttkrp_mode(f, m, F, st.M, s, timeMttkrp ? ev.e[2 * m] : nullptr, timeMttkrp ? ev.e[2 * m + 1] : nullptr);
@@ -576,25 +756,39 @@ static void cpd_als_on(const Format &f, int R, int iters, double tol, int dtype,
                 MMCSF_HIP(hipStreamWaitEvent(s, evInv, 0));
             } else {
                 hipLaunchKernelGGL(k_normal_inverse, dim3(1), dim3(256), ishm, s, st.G, N, m, R, st.W, st.Hinv,
-                                   st.status + m);
+                                   hpad, st.status + m);
             }
-            if (R <= 32 && R % 4 == 0 && reinterpret_cast<uintptr_t>(U[m]) % 16 == 0) {  // 16-byte row stores
-                hipLaunchKernelGGL(k_apply_inverse32, dim3((unsigned)((f.dims[m] + 255) / 256)), dim3(256), 0, s, st.M,
-                                   st.Hinv, (uint32_t)f.dims[m], R, U[m]);
+            const int kind = it == 0 ? 0 : 1;  // 2-norm at iteration 0, max-norm after
+            if (fused) {
+                // two launches: the row solve with the column norms, then lambda,
+                // the normalisation, the bfloat16 copy and the Gram
+                const int h = (int)(upd & 1);
+                hipLaunchKernelGGL(k_solve_rows32, dim3((unsigned)((f.dims[m] + kSolveThreads - 1) / kSolveThreads)),
+                                   dim3(kSolveThreads), 0, s, st.M, st.Hpad, (uint32_t)f.dims[m], R, U[m], kind,
+                                   st.sq + h * R, st.mx + h * R, st.G + m * RR);
+                const uint64_t waves = (f.dims[m] + kScaleRows - 1) / kScaleRows;
+                const uint64_t wavesPerBlock = kScaleThreads / 64;
+                hipLaunchKernelGGL(k_scale_gram32, dim3((unsigned)((waves + wavesPerBlock - 1) / wavesPerBlock)),
+                                   dim3(kScaleThreads), 0, s, U[m],
+                                   (uint32_t)f.dims[m], R, kind, st.sq + h * R, st.mx + h * R, st.lam,
+                                   st.sq + (1 - h) * R, st.mx + (1 - h) * R,
+                                   dtype == MMCSF_BF16 ? st.Ub[m] : (uint16_t *)nullptr, st.G + m * RR);
             } else {
                 const size_t shm = RR * sizeof(double) <= 32768 ? RR * sizeof(double) : 0;
                 hipLaunchKernelGGL(k_apply_inverse, dim3(grid_for(f.dims[m] * ((R + 3) / 4), 8192)), dim3(256), shm, s,
                                    st.M, st.Hinv, (uint32_t)f.dims[m], R, U[m]);
+                const int h = (int)(upd & 1);
+                clear_norms();
+                hipLaunchKernelGGL(k_colnorm, dim3(colnorm_grid(f.dims[m], R)), dim3(256), 0, s, U[m],
+                                   (uint32_t)f.dims[m], R, kind, st.sq + h * R, st.mx + h * R);
+                hipLaunchKernelGGL(k_lambda, dim3(1), dim3(std::max(R, 64)), 0, s, R, kind, st.sq + h * R,
+                                   st.mx + h * R, st.lam);
+                hipLaunchKernelGGL(k_scale_cols, dim3(grid_for(f.dims[m] * R, 8192)), dim3(256), 0, s, U[m],
+                                   (uint32_t)f.dims[m], R, st.lam, (double *)nullptr);
+                gram(U[m], f.dims[m], R, st.G + m * RR, s);
+                refresh_bf16(m);
             }
-            const int kind = it == 0 ? 0 : 1;  // 2-norm at iteration 0, max-norm after
-            clear_norms();
-            hipLaunchKernelGGL(k_colnorm, dim3(colnorm_grid(f.dims[m], R)), dim3(256), 0, s, U[m],
-                               (uint32_t)f.dims[m], R, kind, st.sq, st.mx);
-            hipLaunchKernelGGL(k_lambda, dim3(1), dim3(std::max(R, 64)), 0, s, R, kind, st.sq, st.mx, st.lam);
-            hipLaunchKernelGGL(k_scale_cols, dim3(grid_for(f.dims[m] * R, 8192)), dim3(256), 0, s, U[m],
-                               (uint32_t)f.dims[m], R, st.lam, (double *)nullptr);
-            gram(U[m], f.dims[m], R, st.G + m * RR, s);
-            refresh_bf16(m);
+            ++upd;
             if (m == N - 1) {
                 // inner product of the last MTTKRP with the updated last factor
                 hipLaunchKernelGGL(k_zero_d, dim3(1), dim3(64), 0, s, st.scal + 1, (uint64_t)1);
