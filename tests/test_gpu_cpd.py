@@ -42,8 +42,10 @@ def planted_tensor(rng, dims, nnz, rank, zipf0=1.2, noise=0.01):
 
 
 # R <= 36 factors the normal matrix by Cholesky in LDS, R = 48 in the global
-# work space; R = 5 has a partial column quad in the row solve
-@pytest.mark.parametrize("R", [5, 8, 16, 48])
+# work space; R = 5 has a partial column quad in the row solve.  R <= 32 with
+# R % 4 == 0 takes the fused row solve and normalisation + Gram: one 16-column
+# tile (R = 8, 16), a partial second tile (R = 24), two full tiles (R = 32)
+@pytest.mark.parametrize("R", [5, 8, 16, 24, 32, 48])
 def test_cpd_als_matches_float64_als(cuda, R):
     import torch
     import mmcsf
@@ -88,12 +90,12 @@ def test_cpd_als_singular_normal_equations(cuda):
         mmcsf.cpd_als(fmt, Ud, iters=2, tol=0.0)
 
 
-def test_cpd_als_bf16_mttkrp(cuda):
+@pytest.mark.parametrize("R", [16, 24])
+def test_cpd_als_bf16_mttkrp(cuda, R):
     import torch
     import mmcsf
     rng = np.random.default_rng(3)
     dims = [2000, 1000, 200]
-    R = 16
     inds, vals = planted_tensor(rng, dims, 60000, R)
     U0 = [(rng.random((d, R)) * 6 - 3).astype(np.float32) for d in dims]
     t = mmcsf.Tensor.from_numpy(dims, inds, vals)
