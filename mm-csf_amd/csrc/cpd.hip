@@ -27,6 +27,10 @@ typedef float v4f __attribute__((ext_vector_type(4)));
 // LDS bytes for k_normal_inverse's three R x R float64 matrices (R <= 36)
 constexpr size_t kInverseLds = 32768;
 
+// iterations run between two host reads of the fits and solve statuses when no
+// tolerance can stop the loop early
+constexpr int kFitBatch = 64;
+
 // ---------------------------------------------------------------------------
 // Gram: G += U^T U for U row-major rows x R.  blockIdx.y picks a 16x16 tile
 // pair (bi <= bj); every wave reduces `rowsPerWave` rows with MFMA:
@@ -557,8 +561,11 @@ __global__ __launch_bounds__(256) void k_sumsq(const float *__restrict__ v, uint
 
 // fit = 1 - sqrt(|X|^2 + lam^T (Hadamard_m G_m) lam - 2 inner) / |X|
 // (KruskalTensorFit, cpd_cpu.cu:440-465, with the full symmetric product)
+// fit of the iteration into fit[0] and hist[0]; acc[0] counts the modes whose
+// solve fell back to LU (status 1), acc[1] is set when one was singular (2)
 __global__ void k_fit(const double *__restrict__ G, int N, int R, const double *__restrict__ lam,
-                      const double *__restrict__ inner, const double *__restrict__ xnormsq, double *__restrict__ fit) {
+                      const double *__restrict__ inner, const double *__restrict__ xnormsq, double *__restrict__ fit,
+                      double *__restrict__ hist, const int *__restrict__ status, int *__restrict__ acc) {
     __shared__ double red[256];
     double s = 0.0;
     for (int e = threadIdx.x; e < R * R; e += blockDim.x) {
@@ -577,6 +584,11 @@ __global__ void k_fit(const double *__restrict__ G, int N, int R, const double *
         double resid = xnormsq[0] + normMats - 2.0 * inner[0];
         if (resid > 0.0) resid = sqrt(resid);
         fit[0] = 1.0 - resid / sqrt(xnormsq[0]);
+        hist[0] = fit[0];
+        for (int m = 0; m < N; ++m) {
+            if (status[m] == 1) ++acc[0];
+            if (status[m] == 2) acc[1] = 1;
+        }
     }
 }
 
@@ -627,9 +639,9 @@ __global__ void k_zero_u32(uint32_t *p, uint64_t n) {
 // updated, and (bfloat16 MTTKRP) the bfloat16 factor copies; every part
 // 256-byte aligned.
 struct CpdSpace {
-    double *G, *W, *Hinv, *Hpad, *lam, *lamFinal, *sq, *scal;
+    double *G, *W, *Hinv, *Hpad, *lam, *lamFinal, *sq, *scal, *hist;
     uint32_t *mx;
-    int *status;
+    int *status, *acc;
     float *M;
     uint16_t *Ub[kMaxDims];
 };
@@ -659,6 +671,8 @@ static size_t cpd_space(const Format &f, int R, int dtype, char *base, CpdSpace 
     t.scal = (double *)take(4 * sizeof(double));  // [0] |X|^2, [1] inner, [2] fit
     t.mx = (uint32_t *)take(2 * R * sizeof(uint32_t));
     t.status = (int *)take(kMaxDims * sizeof(int));  // solve status of each mode
+    t.hist = (double *)take(kFitBatch * sizeof(double));  // fits of the iterations since the last host read
+    t.acc = (int *)take(2 * sizeof(int));                 // LU fallbacks, singular flag
     t.M = (float *)take(maxRows * R * sizeof(float));
     for (int m = 0; m < kMaxDims; ++m) t.Ub[m] = nullptr;
     if (dtype == MMCSF_BF16)
@@ -697,6 +711,7 @@ static void cpd_als_on(const Format &f, int R, int iters, double tol, int dtype,
     }
     hipLaunchKernelGGL(k_zero_d, dim3(1), dim3(256), 0, s, st.scal, (uint64_t)4);
     hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, (uint32_t *)st.status, (uint64_t)kMaxDims);
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, (uint32_t *)st.acc, (uint64_t)2);
     // |X|^2 over every stored value (the partitions of a mixed MM-CSF)
     if (f.mixed) {
         for (int p = 0; p < N; ++p)
@@ -712,114 +727,132 @@ static void cpd_als_on(const Format &f, int R, int iters, double tol, int dtype,
     F.dtype = dtype;
     F.R = R;
     for (int m = 0; m < N; ++m) F.U[m] = dtype == MMCSF_BF16 ? (const void *)st.Ub[m] : (const void *)U[m];
-    // events: MTTKRP start / end per mode (timing), and the two joins of the
-    // side stream (after the previous mode's Gram, after the inverse)
+    // The host reads the fits and solve statuses after every iteration when a
+    // tolerance may stop the loop (cpd_cpu.cu:670-672), else once per batch of
+    // up to kFitBatch iterations: a tolerance <= 0 never stops it early.
+    const int batch = tol > 0.0 ? 1 : std::min(kFitBatch, std::max(iters, 1));
+    // events: MTTKRP start / end per iteration of a batch and mode (timing),
+    // and the two joins of the side stream (after the previous mode's Gram,
+    // after the inverse)
     struct Events {
-        hipEvent_t e[2 * kMaxDims + 2] = {};
+        std::vector<hipEvent_t> e;
         ~Events() {
             for (hipEvent_t x : e)
                 if (x) (void)hipEventDestroy(x);
         }
     } ev;
-    if (timeMttkrp)
-        for (int i = 0; i < 2 * N; ++i) MMCSF_HIP(hipEventCreate(&ev.e[i]));
+    const int nTime = timeMttkrp ? 2 * N * batch : 0;
+    ev.e.assign(nTime + 2, nullptr);
+    for (int i = 0; i < nTime; ++i) MMCSF_HIP(hipEventCreate(&ev.e[i]));
     hipEvent_t evReady = nullptr, evInv = nullptr;
     if (side) {
-        MMCSF_HIP(hipEventCreateWithFlags(&ev.e[2 * kMaxDims], hipEventDisableTiming));
-        MMCSF_HIP(hipEventCreateWithFlags(&ev.e[2 * kMaxDims + 1], hipEventDisableTiming));
-        evReady = ev.e[2 * kMaxDims];
-        evInv = ev.e[2 * kMaxDims + 1];
+        MMCSF_HIP(hipEventCreateWithFlags(&ev.e[nTime], hipEventDisableTiming));
+        MMCSF_HIP(hipEventCreateWithFlags(&ev.e[nTime + 1], hipEventDisableTiming));
+        evReady = ev.e[nTime];
+        evInv = ev.e[nTime + 1];
     }
     double mttkrpMs = 0, oldfit = 0, fit = 0;
     int it = 0, lu = 0;
     uint64_t upd = 0;  // factor updates so far (selects the half of the norm accumulators)
     clear_norms();
     auto t0 = std::chrono::steady_clock::now();
-    for (it = 0; it < iters; ++it) {
-        for (int m = 0; m < N; ++m) {
-            const size_t ishm = 3 * RR * sizeof(double) <= kInverseLds ? 3 * RR * sizeof(double) : 0;
-            // the fused row solve (R <= 32, 16-byte rows) reads H^-1 zero-padded
-            const bool fused = R <= 32 && R % 4 == 0 && reinterpret_cast<uintptr_t>(U[m]) % 16 == 0;
-            double *hpad = fused ? st.Hpad : nullptr;
-            if (side) {
-                // the Grams of the other modes are final once the work queued on
-                // s so far (the previous mode's update) is done; the inverse
-                // then runs beside this mode's MTTKRP
-                MMCSF_HIP(hipEventRecord(evReady, s));
-                MMCSF_HIP(hipStreamWaitEvent(side, evReady, 0));
-                hipLaunchKernelGGL(k_normal_inverse, dim3(1), dim3(256), ishm, side, st.G, N, m, R, st.W, st.Hinv,
-                                   hpad, st.status + m);
-                MMCSF_HIP(hipEventRecord(evInv, side));
+    bool stop = false;
+    while (it < iters && !stop) {
+        const int nb = std::min(batch, iters - it);
+        for (int b = 0; b < nb; ++b) {
+            const int cur = it + b;
+            for (int m = 0; m < N; ++m) {
+                const size_t ishm = 3 * RR * sizeof(double) <= kInverseLds ? 3 * RR * sizeof(double) : 0;
+                // the fused row solve (R <= 32, 16-byte rows) reads H^-1 zero-padded
+                const bool fused = R <= 32 && R % 4 == 0 && reinterpret_cast<uintptr_t>(U[m]) % 16 == 0;
+                double *hpad = fused ? st.Hpad : nullptr;
+                if (side) {
+                    // the Grams of the other modes are final once the work queued on
+                    // s so far (the previous mode's update) is done; the inverse
+                    // then runs beside this mode's MTTKRP
+                    MMCSF_HIP(hipEventRecord(evReady, s));
+                    MMCSF_HIP(hipStreamWaitEvent(side, evReady, 0));
+                    hipLaunchKernelGGL(k_normal_inverse, dim3(1), dim3(256), ishm, side, st.G, N, m, R, st.W, st.Hinv,
+                                       hpad, st.status + m);
+                    MMCSF_HIP(hipEventRecord(evInv, side));
+                }
+                hipEvent_t *te = timeMttkrp ? &ev.e[2 * (b * N + m)] : nullptr;
+                mttkrp_mode(f, m, F, st.M, s, te ? te[0] : nullptr, te ? te[1] : nullptr);
+                if (side) {
+                    MMCSF_HIP(hipStreamWaitEvent(s, evInv, 0));
+                } else {
+                    hipLaunchKernelGGL(k_normal_inverse, dim3(1), dim3(256), ishm, s, st.G, N, m, R, st.W, st.Hinv,
+                                       hpad, st.status + m);
+                }
+                const int kind = cur == 0 ? 0 : 1;  // 2-norm at iteration 0, max-norm after
+                if (fused) {
+                    // two launches: the row solve with the column norms, then lambda,
+                    // the normalisation, the bfloat16 copy and the Gram
+                    const int h = (int)(upd & 1);
+                    const unsigned solveBlocks = (unsigned)((f.dims[m] + kSolveThreads - 1) / kSolveThreads);
+                    hipLaunchKernelGGL(k_solve_rows32, dim3(solveBlocks), dim3(kSolveThreads), 0, s, st.M, st.Hpad,
+                                       (uint32_t)f.dims[m], R, U[m], kind, st.sq + h * R, st.mx + h * R,
+                                       st.G + m * RR);
+                    const uint64_t waves = (f.dims[m] + kScaleRows - 1) / kScaleRows;
+                    const uint64_t wavesPerBlock = kScaleThreads / 64;
+                    hipLaunchKernelGGL(k_scale_gram32, dim3((unsigned)((waves + wavesPerBlock - 1) / wavesPerBlock)),
+                                       dim3(kScaleThreads), 0, s, U[m],
+                                       (uint32_t)f.dims[m], R, kind, st.sq + h * R, st.mx + h * R, st.lam,
+                                       st.sq + (1 - h) * R, st.mx + (1 - h) * R,
+                                       dtype == MMCSF_BF16 ? st.Ub[m] : (uint16_t *)nullptr, st.G + m * RR);
+                } else {
+                    const size_t shm = RR * sizeof(double) <= 32768 ? RR * sizeof(double) : 0;
+                    hipLaunchKernelGGL(k_apply_inverse, dim3(grid_for(f.dims[m] * ((R + 3) / 4), 8192)), dim3(256),
+                                       shm, s, st.M, st.Hinv, (uint32_t)f.dims[m], R, U[m]);
+                    const int h = (int)(upd & 1);
+                    clear_norms();
+                    hipLaunchKernelGGL(k_colnorm, dim3(colnorm_grid(f.dims[m], R)), dim3(256), 0, s, U[m],
+                                       (uint32_t)f.dims[m], R, kind, st.sq + h * R, st.mx + h * R);
+                    hipLaunchKernelGGL(k_lambda, dim3(1), dim3(std::max(R, 64)), 0, s, R, kind, st.sq + h * R,
+                                       st.mx + h * R, st.lam);
+                    hipLaunchKernelGGL(k_scale_cols, dim3(grid_for(f.dims[m] * R, 8192)), dim3(256), 0, s, U[m],
+                                       (uint32_t)f.dims[m], R, st.lam, (double *)nullptr);
+                    gram(U[m], f.dims[m], R, st.G + m * RR, s);
+                    refresh_bf16(m);
+                }
+                ++upd;
+                if (m == N - 1) {
+                    // inner product of the last MTTKRP with the updated last factor
+                    hipLaunchKernelGGL(k_zero_d, dim3(1), dim3(64), 0, s, st.scal + 1, (uint64_t)1);
+                    hipLaunchKernelGGL(k_dot_lambda, dim3(grid_for(f.dims[m] * R, 512)), dim3(256), 0, s, U[m], st.M,
+                                       (uint32_t)f.dims[m], R, st.lam, st.scal + 1);
+                }
+                MMCSF_HIP(hipGetLastError());
             }
-            mttkrp_mode(f, m, F, st.M, s, timeMttkrp ? ev.e[2 * m] : nullptr, timeMttkrp ? ev.e[2 * m + 1] : nullptr);
-            if (side) {
-                MMCSF_HIP(hipStreamWaitEvent(s, evInv, 0));
-            } else {
-                hipLaunchKernelGGL(k_normal_inverse, dim3(1), dim3(256), ishm, s, st.G, N, m, R, st.W, st.Hinv,
-                                   hpad, st.status + m);
-            }
-            const int kind = it == 0 ? 0 : 1;  // 2-norm at iteration 0, max-norm after
-            if (fused) {
-                // two launches: the row solve with the column norms, then lambda,
-                // the normalisation, the bfloat16 copy and the Gram
-                const int h = (int)(upd & 1);
-                hipLaunchKernelGGL(k_solve_rows32, dim3((unsigned)((f.dims[m] + kSolveThreads - 1) / kSolveThreads)),
-                                   dim3(kSolveThreads), 0, s, st.M, st.Hpad, (uint32_t)f.dims[m], R, U[m], kind,
-                                   st.sq + h * R, st.mx + h * R, st.G + m * RR);
-                const uint64_t waves = (f.dims[m] + kScaleRows - 1) / kScaleRows;
-                const uint64_t wavesPerBlock = kScaleThreads / 64;
-                hipLaunchKernelGGL(k_scale_gram32, dim3((unsigned)((waves + wavesPerBlock - 1) / wavesPerBlock)),
-                                   dim3(kScaleThreads), 0, s, U[m],
-                                   (uint32_t)f.dims[m], R, kind, st.sq + h * R, st.mx + h * R, st.lam,
-                                   st.sq + (1 - h) * R, st.mx + (1 - h) * R,
-                                   dtype == MMCSF_BF16 ? st.Ub[m] : (uint16_t *)nullptr, st.G + m * RR);
-            } else {
-                const size_t shm = RR * sizeof(double) <= 32768 ? RR * sizeof(double) : 0;
-                hipLaunchKernelGGL(k_apply_inverse, dim3(grid_for(f.dims[m] * ((R + 3) / 4), 8192)), dim3(256), shm, s,
-                                   st.M, st.Hinv, (uint32_t)f.dims[m], R, U[m]);
-                const int h = (int)(upd & 1);
-                clear_norms();
-                hipLaunchKernelGGL(k_colnorm, dim3(colnorm_grid(f.dims[m], R)), dim3(256), 0, s, U[m],
-                                   (uint32_t)f.dims[m], R, kind, st.sq + h * R, st.mx + h * R);
-                hipLaunchKernelGGL(k_lambda, dim3(1), dim3(std::max(R, 64)), 0, s, R, kind, st.sq + h * R,
-                                   st.mx + h * R, st.lam);
-                hipLaunchKernelGGL(k_scale_cols, dim3(grid_for(f.dims[m] * R, 8192)), dim3(256), 0, s, U[m],
-                                   (uint32_t)f.dims[m], R, st.lam, (double *)nullptr);
-                gram(U[m], f.dims[m], R, st.G + m * RR, s);
-                refresh_bf16(m);
-            }
-            ++upd;
-            if (m == N - 1) {
-                // inner product of the last MTTKRP with the updated last factor
-                hipLaunchKernelGGL(k_zero_d, dim3(1), dim3(64), 0, s, st.scal + 1, (uint64_t)1);
-                hipLaunchKernelGGL(k_dot_lambda, dim3(grid_for(f.dims[m] * R, 512)), dim3(256), 0, s, U[m], st.M,
-                                   (uint32_t)f.dims[m], R, st.lam, st.scal + 1);
-            }
-            MMCSF_HIP(hipGetLastError());
+            hipLaunchKernelGGL(k_fit, dim3(1), dim3(256), 0, s, st.G, N, R, st.lam, st.scal + 1, st.scal, st.scal + 2,
+                               st.hist + b, st.status, st.acc);
         }
-        hipLaunchKernelGGL(k_fit, dim3(1), dim3(256), 0, s, st.G, N, R, st.lam, st.scal + 1, st.scal, st.scal + 2);
-        int stat[kMaxDims] = {0, 0, 0, 0};
-        MMCSF_HIP(hipMemcpyAsync(&fit, st.scal + 2, sizeof(double), hipMemcpyDeviceToHost, s));
-        MMCSF_HIP(hipMemcpyAsync(stat, st.status, N * sizeof(int), hipMemcpyDeviceToHost, s));
+        double fits[kFitBatch];
+        int acc[2] = {0, 0};
+        MMCSF_HIP(hipMemcpyAsync(fits, st.hist, nb * sizeof(double), hipMemcpyDeviceToHost, s));
+        MMCSF_HIP(hipMemcpyAsync(acc, st.acc, sizeof(acc), hipMemcpyDeviceToHost, s));
         MMCSF_HIP(hipStreamSynchronize(s));
         // a singular solve leaves that mode's factor (and the later modes')
         // undefined: the call fails
-        for (int m = 0; m < N; ++m) {
-            if (stat[m] == 1) ++lu;
-            MMCSF_REQUIRE(stat[m] != 2, MMCSF_ERR_UNSUPPORTED, "normal equations are singular");
-        }
-        if (timeMttkrp)
-            for (int m = 0; m < N; ++m) {
-                float ms = 0;
-                MMCSF_HIP(hipEventElapsedTime(&ms, ev.e[2 * m], ev.e[2 * m + 1]));
-                mttkrpMs += ms;
+        MMCSF_REQUIRE(acc[1] == 0, MMCSF_ERR_UNSUPPORTED, "normal equations are singular");
+        lu = acc[0];
+        for (int b = 0; b < nb && !stop; ++b) {
+            const int cur = it + b;
+            if (timeMttkrp)
+                for (int m = 0; m < N; ++m) {
+                    float ms = 0;
+                    MMCSF_HIP(hipEventElapsedTime(&ms, ev.e[2 * (b * N + m)], ev.e[2 * (b * N + m) + 1]));
+                    mttkrpMs += ms;
+                }
+            fit = fits[b];
+            if (stats && cur < (int)(sizeof(stats->fit_history) / sizeof(double))) stats->fit_history[cur] = fit;
+            if (cur > 0 && std::fabs(fit - oldfit) < tol) {  // cpd_cpu.cu:670-672
+                it = cur + 1;
+                stop = true;
             }
-        if (stats && it < (int)(sizeof(stats->fit_history) / sizeof(double))) stats->fit_history[it] = fit;
-        if (it > 0 && std::fabs(fit - oldfit) < tol) {  // cpd_cpu.cu:670-672
-            ++it;
-            break;
+            oldfit = fit;
         }
-        oldfit = fit;
+        if (!stop) it += nb;
     }
     auto t1 = std::chrono::steady_clock::now();
     // GetFinalLambda (cpd_cpu.cu:333-349): fold the column 2-norms into lambda
