@@ -1,7 +1,8 @@
 // CP-ALS on the GPU (cpd() of the reference, src/cpd_cpu.cu:484-691).
 //
-// Everything stays on the device; the host reads one float64 (the fit) per
-// iteration.  Arithmetic per step:
+// Everything stays on the device; the host reads the fits (one float64 per
+// iteration) and the solve statuses, after each iteration when a tolerance can
+// stop the loop, else in batches.  Arithmetic per step:
 //   MTTKRP                 the MM-CSF kernels (float32 accumulation)
 //   Gram  A^T A            MFMA v_mfma_f32_16x16x4f32 over row chunks, chunk sums
 //                          reduced in float64
@@ -686,8 +687,9 @@ static size_t cpd_space(const Format &f, int R, int dtype, char *base, CpdSpace 
 // anything on entry.  timeMttkrp: time the MTTKRP launches with HIP events.
 // side (optional, non-blocking): a second stream on which each mode's normal
 // matrix is inverted while that mode's MTTKRP runs (the inverse needs only the
-// other modes' Grams); null runs everything on s.  The host reads the modes'
-// solve status and the fit once per iteration.
+// other modes' Grams); null runs everything on s.  The host reads the fits
+// and solve statuses after every iteration when tol > 0, else once per batch
+// of up to kFitBatch iterations.
 static void cpd_als_on(const Format &f, int R, int iters, double tol, int dtype, float *const *U, double *lambdaHost,
                        mmcsf_cpd_stats *stats, hipStream_t s, char *ws, bool timeMttkrp, hipStream_t side) {
     const int N = f.ndims;
