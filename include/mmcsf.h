@@ -351,7 +351,11 @@ typedef struct mmcsf_cpd_stats {
  * inverse (LU fallback), float64 row solve, column 2-norm (iteration 0) or
  * max-norm, MFMA Gram of the new factor.  The fit (float64) is
  * 1 - |X - [[lambda; U]]|_F / |X|_F; the loop stops after max_iters or when
- * the fit changes by less than tol.
+ * the fit changes by less than tol.  A normal matrix that neither Cholesky nor
+ * LU can factor fails the call (MMCSF_ERR_UNSUPPORTED, "singular").  The host
+ * reads the fits after every iteration when tol > 0; with tol <= 0 (no early
+ * stop) it reads them once per 64 iterations, and a singular solve is then
+ * reported at the end of its batch.
  */
 int mmcsf_cpd_als(const mmcsf_format *f, int R, int max_iters, double tol,
                   int mttkrp_dtype, float *const *factors, double *lambda,
