@@ -354,15 +354,25 @@ __device__ inline uint16_t to_bf16(float x) {
 // 16-31 x 16-31, MFMA operands laid out as in k_gram), so each element is
 // read, scaled and written by one lane.  The block's 16 waves are summed in
 // LDS before the block adds its tiles to G.  Block 0 writes lambda and clears
-// sqNext / mxNext, the norm accumulators of the next update.
+// sqNext / mxNext, the norm accumulators of the next update.  Mz (optional):
+// the first nz floats of the MTTKRP output, cleared for the next mode's
+// MTTKRP (nothing reads them after the row solve).
 constexpr int kScaleThreads = 1024;
 constexpr int kScaleRows = 64;
 __global__ __launch_bounds__(kScaleThreads) void k_scale_gram32(float *__restrict__ U, uint32_t rows, int R,
                                                       int kind, const double *__restrict__ sq,
                                                       const uint32_t *__restrict__ mx, double *__restrict__ lam,
                                                       double *__restrict__ sqNext, uint32_t *__restrict__ mxNext,
-                                                      uint16_t *__restrict__ Ub, double *__restrict__ G) {
+                                                      uint16_t *__restrict__ Ub, double *__restrict__ G,
+                                                      float *__restrict__ Mz, uint64_t nz) {
     constexpr int kWaves = kScaleThreads / 64;
+    if (Mz) {
+        float4 *z4 = reinterpret_cast<float4 *>(Mz);
+        const uint64_t n4 = nz / 4, stride = (uint64_t)gridDim.x * blockDim.x;
+        for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n4; i += stride)
+            z4[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+        for (uint64_t i = n4 * 4 + blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < nz; i += stride) Mz[i] = 0.f;
+    }
     __shared__ float part[3][kWaves][256];
     if (blockIdx.x == 0 && threadIdx.x < R) {
         lam[threadIdx.x] = lambda_of(kind, sq, mx, threadIdx.x);
@@ -563,9 +573,10 @@ __global__ __launch_bounds__(256) void k_sumsq(const float *__restrict__ v, uint
 // fit = 1 - sqrt(|X|^2 + lam^T (Hadamard_m G_m) lam - 2 inner) / |X|
 // (KruskalTensorFit, cpd_cpu.cu:440-465, with the full symmetric product)
 // fit of the iteration into fit[0] and hist[0]; acc[0] counts the modes whose
-// solve fell back to LU (status 1), acc[1] is set when one was singular (2)
+// solve fell back to LU (status 1), acc[1] is set when one was singular (2).
+// inner[0] is cleared once read: k_dot_lambda of the next iteration adds to it.
 __global__ void k_fit(const double *__restrict__ G, int N, int R, const double *__restrict__ lam,
-                      const double *__restrict__ inner, const double *__restrict__ xnormsq, double *__restrict__ fit,
+                      double *__restrict__ inner, const double *__restrict__ xnormsq, double *__restrict__ fit,
                       double *__restrict__ hist, const int *__restrict__ status, int *__restrict__ acc) {
     __shared__ double red[256];
     double s = 0.0;
@@ -583,6 +594,7 @@ __global__ void k_fit(const double *__restrict__ G, int N, int R, const double *
     if (threadIdx.x == 0) {
         double normMats = fabs(red[0]);
         double resid = xnormsq[0] + normMats - 2.0 * inner[0];
+        inner[0] = 0.0;
         if (resid > 0.0) resid = sqrt(resid);
         fit[0] = 1.0 - resid / sqrt(xnormsq[0]);
         hist[0] = fit[0];
@@ -756,6 +768,7 @@ static void cpd_als_on(const Format &f, int R, int iters, double tol, int dtype,
     double mttkrpMs = 0, oldfit = 0, fit = 0;
     int it = 0, lu = 0;
     uint64_t upd = 0;  // factor updates so far (selects the half of the norm accumulators)
+    bool mZeroed = false;  // the MTTKRP output was cleared for the next mode by the previous update
     clear_norms();
     auto t0 = std::chrono::steady_clock::now();
     bool stop = false;
@@ -779,7 +792,7 @@ static void cpd_als_on(const Format &f, int R, int iters, double tol, int dtype,
                     MMCSF_HIP(hipEventRecord(evInv, side));
                 }
                 hipEvent_t *te = timeMttkrp ? &ev.e[2 * (b * N + m)] : nullptr;
-                mttkrp_mode(f, m, F, st.M, s, te ? te[0] : nullptr, te ? te[1] : nullptr);
+                mttkrp_mode(f, m, F, st.M, s, te ? te[0] : nullptr, te ? te[1] : nullptr, mZeroed);
                 if (side) {
                     MMCSF_HIP(hipStreamWaitEvent(s, evInv, 0));
                 } else {
@@ -795,13 +808,19 @@ static void cpd_als_on(const Format &f, int R, int iters, double tol, int dtype,
                     hipLaunchKernelGGL(k_solve_rows32, dim3(solveBlocks), dim3(kSolveThreads), 0, s, st.M, st.Hpad,
                                        (uint32_t)f.dims[m], R, U[m], kind, st.sq + h * R, st.mx + h * R,
                                        st.G + m * RR);
+                    // modes before the last also clear the next mode's MTTKRP output
+                    // (the last mode's output feeds the inner product of the fit)
+                    const bool clearNext = m + 1 < N;
                     const uint64_t waves = (f.dims[m] + kScaleRows - 1) / kScaleRows;
                     const uint64_t wavesPerBlock = kScaleThreads / 64;
                     hipLaunchKernelGGL(k_scale_gram32, dim3((unsigned)((waves + wavesPerBlock - 1) / wavesPerBlock)),
                                        dim3(kScaleThreads), 0, s, U[m],
                                        (uint32_t)f.dims[m], R, kind, st.sq + h * R, st.mx + h * R, st.lam,
                                        st.sq + (1 - h) * R, st.mx + (1 - h) * R,
-                                       dtype == MMCSF_BF16 ? st.Ub[m] : (uint16_t *)nullptr, st.G + m * RR);
+                                       dtype == MMCSF_BF16 ? st.Ub[m] : (uint16_t *)nullptr, st.G + m * RR,
+                                       clearNext ? st.M : (float *)nullptr,
+                                       clearNext ? f.dims[m + 1] * (uint64_t)R : 0ull);
+                    mZeroed = clearNext;
                 } else {
                     const size_t shm = RR * sizeof(double) <= 32768 ? RR * sizeof(double) : 0;
                     hipLaunchKernelGGL(k_apply_inverse, dim3(grid_for(f.dims[m] * ((R + 3) / 4), 8192)), dim3(256),
@@ -816,11 +835,12 @@ static void cpd_als_on(const Format &f, int R, int iters, double tol, int dtype,
                                        (uint32_t)f.dims[m], R, st.lam, (double *)nullptr);
                     gram(U[m], f.dims[m], R, st.G + m * RR, s);
                     refresh_bf16(m);
+                    mZeroed = false;
                 }
                 ++upd;
                 if (m == N - 1) {
                     // inner product of the last MTTKRP with the updated last factor
-                    hipLaunchKernelGGL(k_zero_d, dim3(1), dim3(64), 0, s, st.scal + 1, (uint64_t)1);
+                    // (its accumulator is cleared by the setup and by k_fit)
                     hipLaunchKernelGGL(k_dot_lambda, dim3(grid_for(f.dims[m] * R, 512)), dim3(256), 0, s, U[m], st.M,
                                        (uint32_t)f.dims[m], R, st.lam, st.scal + 1);
                 }

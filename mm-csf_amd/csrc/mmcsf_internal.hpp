@@ -215,13 +215,15 @@ struct Factors {
 // evStart / evEnd (optional) bracket the MTTKRP kernel only, after the
 // output has been zeroed.
 // accumulate: add to `out` (not zeroed here; every row written with atomics)
+// zeroed: `out` already holds zeros (the caller cleared it); not zeroed again
 void mttkrp_csf(const Csf &c, const uint64_t *dims, int outLevel, const Factors &F, float *out,
-                hipStream_t s, hipEvent_t evStart = nullptr, hipEvent_t evEnd = nullptr, bool accumulate = false);
+                hipStream_t s, hipEvent_t evStart = nullptr, hipEvent_t evEnd = nullptr, bool accumulate = false,
+                bool zeroed = false);
 void mttkrp_coo(const Format &f, const Factors &F, float *out, hipStream_t s);
 // MTTKRP of tensor mode `mode` of a CSF-based format (types 8 and 12),
 // through the mode's root tree when the format has one
 void mttkrp_mode(const Format &f, int mode, const Factors &F, float *out, hipStream_t s,
-                 hipEvent_t evStart = nullptr, hipEvent_t evEnd = nullptr);
+                 hipEvent_t evStart = nullptr, hipEvent_t evEnd = nullptr, bool zeroed = false);
 // root-level MTTKRP over 64-nonzero chunks (walk.hip); needs R % 4 == 0 and
 // the start bitmasks of the tree
 bool walk_supports(const Csf &c, int R);

@@ -490,9 +490,9 @@ void fill_zero(float *p, uint64_t n, hipStream_t s) {
 }
 
 void mttkrp_csf(const Csf &c, const uint64_t *dims, int outLevel, const Factors &F, float *out,
-                hipStream_t s, hipEvent_t evStart, hipEvent_t evEnd, bool accumulate) {
+                hipStream_t s, hipEvent_t evStart, hipEvent_t evEnd, bool accumulate, bool zeroed) {
     const int N = c.ndims;
-    if (!accumulate) fill_zero(out, dims[c.order[outLevel]] * (uint64_t)F.R, s);
+    if (!accumulate && !zeroed) fill_zero(out, dims[c.order[outLevel]] * (uint64_t)F.R, s);
     if (evStart) MMCSF_HIP(hipEventRecord(evStart, s));
     struct EndMark {
         hipEvent_t e;
@@ -556,11 +556,11 @@ void mttkrp_csf(const Csf &c, const uint64_t *dims, int outLevel, const Factors 
 }
 
 void mttkrp_mode(const Format &f, int mode, const Factors &F, float *out, hipStream_t s, hipEvent_t evStart,
-                 hipEvent_t evEnd) {
+                 hipEvent_t evEnd, bool zeroed) {
     if (f.mixed) {
         // every partition adds its nonzeros' contributions, with the kernel of
         // the level the mode takes in its tree (mttkrp_gpu.cu:3610-3719)
-        fill_zero(out, f.dims[mode] * (uint64_t)F.R, s);
+        if (!zeroed) fill_zero(out, f.dims[mode] * (uint64_t)F.R, s);
         if (evStart) MMCSF_HIP(hipEventRecord(evStart, s));
         for (int p = 0; p < f.ndims; ++p) {
             if (!f.partNnz[p]) continue;
@@ -573,13 +573,13 @@ void mttkrp_mode(const Format &f, int mode, const Factors &F, float *out, hipStr
         return;
     }
     if (f.hasTree[mode]) {
-        mttkrp_csf(f.tree[mode], f.dims, 0, F, out, s, evStart, evEnd);
+        mttkrp_csf(f.tree[mode], f.dims, 0, F, out, s, evStart, evEnd, false, zeroed);
         return;
     }
     int lvl = 0;
     for (int l = 0; l < f.ndims; ++l)
         if (f.csf.order[l] == mode) lvl = l;
-    mttkrp_csf(f.csf, f.dims, lvl, F, out, s, evStart, evEnd);
+    mttkrp_csf(f.csf, f.dims, lvl, F, out, s, evStart, evEnd, false, zeroed);
 }
 
 void mttkrp_coo(const Format &f, const Factors &F, float *out, hipStream_t s) {
